@@ -1,0 +1,641 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE (the oracle). Not part of the product.
+ *
+ * Plain-C, OpenMP, FP64 restatement of the reference's PDDSparse CPU path.
+ * See oracle.h for the list of functions and how each one is pinned.
+ */
+#include "oracle.h"
+#include "orc_math.h"
+#include <stdlib.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* elementary pieces                                                         */
+/* ------------------------------------------------------------------------ */
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    orc_philox4x32_10(ctr, key, out);
+}
+
+void orc_math_eval(int fn, const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: y[i] = orc_exp(x[i]); break;
+            case 1: y[i] = orc_log(x[i]); break;
+            case 2: y[i] = orc_sin(x[i]); break;
+            case 3: y[i] = orc_cos(x[i]); break;
+            case 4: y[i] = orc_tanh(x[i]); break;
+            case 5: y[i] = orc_sinpi(x[i]); break;
+            case 6: y[i] = orc_cospi(x[i]); break;
+            default: y[i] = orc_expm1(x[i]); break;
+        }
+    }
+}
+
+/* Normal pair for step `step` of path `path` of knot `knot`.
+   Keying: key = (seed, stream), counter = (call, path, knot, 0).
+   normals == 0: call = step, both normals from one 4-word draw (53-bit uniforms).
+   normals == 1: call = step/2, words (0,1) feed even steps, (2,3) odd steps (24-bit uniforms).
+   Replaces Increment_Update (Integrators/FKACIntegrator.hpp:15-19). */
+static inline void orc_normal_pair(const orc_run *run, uint32_t knot, uint32_t path, int64_t step,
+                                   double *n1, double *n2) {
+    uint32_t key[2] = {run->seed, run->stream};
+    uint32_t w[4];
+    if (run->normals == 0) {
+        uint32_t ctr[4] = {(uint32_t)step, path, knot, (uint32_t)(step >> 32)};
+        orc_philox4x32_10(ctr, key, w);
+        orc_box_muller(orc_u01_53(w[0], w[1]), orc_u01_53(w[2], w[3]), n1, n2);
+    } else {
+        int64_t call = step >> 1;
+        uint32_t ctr[4] = {(uint32_t)call, path, knot, (uint32_t)(call >> 32)};
+        orc_philox4x32_10(ctr, key, w);
+        if ((step & 1) == 0) orc_box_muller(orc_u01_24(w[0]), orc_u01_24(w[1]), n1, n2);
+        else orc_box_muller(orc_u01_24(w[2]), orc_u01_24(w[3]), n1, n2);
+    }
+}
+
+void orc_normals(const orc_run *run, uint32_t knot, uint32_t path, int64_t step0, int64_t nsteps,
+                 double *n1, double *n2) {
+    for (int64_t s = 0; s < nsteps; ++s) orc_normal_pair(run, knot, path, step0 + s, &n1[s], &n2[s]);
+}
+
+/* Signed distance to the rectangle p = [x0,y0,x1,y1], nearest-side projection and
+   outward normal.  Restates Rectangle2D (BVPs/Domains/rectangle.hpp:7-229) with the
+   same operation order (centre/half-side form, quadrant split, inside tie rule
+   "dx-hx > dy-hy", corner normal 0.70710678118f). */
+double orc_rect_dist(const double p[4], double x, double y, double *ex, double *ey, double *nx, double *ny) {
+    const double cx = 0.5 * (p[0] + p[2]), cy = 0.5 * (p[1] + p[3]);
+    const double hx = p[2] - cx, hy = p[3] - cy;
+    const double dx = fabs(x - cx), dy = fabs(y - cy);
+    const double diag = (double)0.70710678118f;
+    const double sxs = (x > cx) ? 1.0 : -1.0;  /* right half: quadrants 1,4 */
+    const double sys = (y > cy) ? 1.0 : -1.0;  /* top half: quadrants 1,2 */
+    *ex = x; *ey = y;
+    if (dx <= hx && dy <= hy) {
+        if (dx - hx > dy - hy) {
+            *ex = sxs * hx + cx; *nx = sxs; *ny = 0.0;
+            return dx - hx;
+        }
+        *ey = sys * hy + cy; *nx = 0.0; *ny = sys;
+        return dy - hy;
+    }
+    if (dx > hx) {
+        if (dy > hy) {
+            double r = sqrt((dx - hx) * (dx - hx) + (dy - hy) * (dy - hy));
+            *ex = sxs * hx + cx; *ey = sys * hy + cy;
+            *nx = sxs * diag; *ny = sys * diag;
+            return r;
+        }
+        *ex = sxs * hx + cx; *nx = sxs; *ny = 0.0;
+        return dx - hx;
+    }
+    *ey = sys * hy + cy; *nx = 0.0; *ny = sys;
+    return dy - hy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PDE coefficients (reference convention, BVPs/BVP.hpp:89-137)              */
+/* ------------------------------------------------------------------------ */
+#define MK1 1.0
+#define MK2 0.01
+#define MK3 0.02
+#define MK4 0.12
+#define MK5 0.05
+#define MK6 0.05
+#define MK7 (-0.12)
+#define MSF 0.333
+
+/* Monegros Poisson: u, grad u, Laplacian (BVPs/EDPs/Monegros_Poisson_CPP.hpp:12-40, 57-59) */
+static void orc_monegros(double x, double y, double *u, double *gx, double *gy, double *lap) {
+    double a1 = sqrt(MK1 + MK2 * x * x + MK3 * y * y);
+    double s1 = orc_sin(MK4 * x + MK5 * y), s2 = orc_sin(MK6 * x + MK7 * y);
+    double c1 = orc_cos(MK4 * x + MK5 * y), c2 = orc_cos(MK6 * x + MK7 * y);
+    double T = orc_tanh(s1 + s2);
+    double sa = orc_sin(a1), ca = orc_cos(a1);
+    double T21 = T * T - 1.0;
+    double px = MK4 * c1 + MK6 * c2, py = MK5 * c1 + MK7 * c2;
+    double a3 = a1 * a1 * a1;
+    *u = 3.0 + MSF * (sa + T);
+    *gx = MSF * ((ca * MK2 * x / a1) - px * T21);
+    *gy = MSF * (ca * MK1 * MK3 * y / a1 - py * T21);
+    double dxx = MSF * (((MK2 / a1) - (MK2 * MK2 * x * x / a3)) * ca - (x * x * MK2 * MK2 / (a1 * a1)) * sa
+                        + 2.0 * px * px * T21 * T + (MK4 * MK4 * s1 + MK6 * MK6 * s2) * T21);
+    double dyy = MSF * (((MK3 / a1) - (MK3 * MK3 * y * y / a3)) * ca - (y * y * MK3 * MK3 / (a1 * a1)) * sa
+                        + 2.0 * py * py * T21 * T + (MK5 * MK5 * s1 + MK7 * MK7 * s2) * T21);
+    *lap = dxx + dyy;
+}
+
+double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
+    const int nx = pb->lut_nx, ny = pb->lut_ny, nf = pb->lut_nf;
+    double t = (x - pb->lut_ox) * pb->lut_sx;
+    double v = (y - pb->lut_oy) * pb->lut_sy;
+    t = fmin(fmax(t, -2.0), (double)nx + 1.0);
+    v = fmin(fmax(v, -2.0), (double)ny + 1.0);
+    double fi = floor(t), fj = floor(v);
+    double fx = t - fi, fy = v - fj;
+    int i = (int)fi, j = (int)fj;
+    double wx[4], wy[4];
+    {
+        double f2 = fx * fx, f3 = f2 * fx, om = 1.0 - fx;
+        wx[0] = (om * om * om) * (1.0 / 6.0);
+        wx[1] = (3.0 * f3 - 6.0 * f2 + 4.0) * (1.0 / 6.0);
+        wx[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * fx + 1.0) * (1.0 / 6.0);
+        wx[3] = f3 * (1.0 / 6.0);
+    }
+    {
+        double f2 = fy * fy, f3 = f2 * fy, om = 1.0 - fy;
+        wy[0] = (om * om * om) * (1.0 / 6.0);
+        wy[1] = (3.0 * f3 - 6.0 * f2 + 4.0) * (1.0 / 6.0);
+        wy[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * fy + 1.0) * (1.0 / 6.0);
+        wy[3] = f3 * (1.0 / 6.0);
+    }
+    double acc = 0.0;
+    for (int b = 0; b < 4; ++b) {
+        int jj = j - 1 + b; jj = jj < 0 ? 0 : (jj > ny - 1 ? ny - 1 : jj);
+        double row = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            int ii = i - 1 + a; ii = ii < 0 ? 0 : (ii > nx - 1 ? nx - 1 : ii);
+            row = row + wx[a] * (double)pb->lut[((int64_t)jj * nx + ii) * nf + field];
+        }
+        acc = acc + wy[b] * row;
+    }
+    return acc;
+}
+
+/* Coefficients at (x,y): sigma (row-major s00 s01 s10 s11), b, c, f, grad u, g, u */
+void orc_pde_eval(const orc_problem *pb, double x, double y, double sig[4], double b[2],
+                  double *c, double *f, double grad[2], double *g, double *u) {
+    switch (pb->pde) {
+        case ORC_PDE_MONEGROS: {
+            double uu, gx, gy, lap;
+            orc_monegros(x, y, &uu, &gx, &gy, &lap);
+            sig[0] = 1.41421356237; sig[1] = 0.0; sig[2] = 0.0; sig[3] = 1.41421356237;
+            b[0] = 0.0; b[1] = 0.0; *c = -0.0; *f = -lap;
+            grad[0] = gx; grad[1] = gy; *g = uu; *u = uu;
+            break;
+        }
+        case ORC_PDE_SINSIN:
+        case ORC_PDE_HELMHOLTZ: {
+            double sx, cx, sy, cy;
+            orc_sincospi(x, &sx, &cx);
+            orc_sincospi(y, &sy, &cy);
+            double uu = sx * sy + x * x + y * y;
+            sig[0] = 1.4142135623730951; sig[1] = 0.0; sig[2] = 0.0; sig[3] = 1.4142135623730951;
+            b[0] = 0.0; b[1] = 0.0;
+            grad[0] = ORC_PI * cx * sy + 2.0 * x;
+            grad[1] = ORC_PI * sx * cy + 2.0 * y;
+            double flap = 19.739208802178716 * sx * sy - 4.0;  /* -Laplacian */
+            if (pb->pde == ORC_PDE_SINSIN) { *c = 0.0; *f = flap; }
+            else { *c = -pb->k2; *f = flap + pb->k2 * uu; }
+            *g = uu; *u = uu;
+            break;
+        }
+        default: { /* VARCOEF and VARCOEF_LUT */
+            double sxn = orc_sin(x), cxn = orc_cos(x), syn = orc_sin(y), cyn = orc_cos(y);
+            double ex = orc_exp(x);
+            double s1 = 1.0 + 0.3 * sxn, s2 = 1.0 + 0.3 * cyn;
+            double uu = ex * cyn, uy = -ex * syn;
+            double a1 = s1 * s1, a2 = s2 * s2;
+            double a1p = 2.0 * s1 * (0.3 * cxn), a2p = 2.0 * s2 * (-0.3 * syn);
+            double fcfg = 0.5 * (a1p * uu + a1 * uu + a2p * uy - a2 * uu)
+                          + 0.2 * y * uu + (-0.2 * x) * uy - (1.0 + x * x) * uu;
+            *f = -fcfg;
+            grad[0] = uu; grad[1] = uy; *g = uu; *u = uu;
+            if (pb->pde == ORC_PDE_VARCOEF) {
+                sig[0] = s1; sig[1] = 0.0; sig[2] = 0.0; sig[3] = s2;
+                b[0] = 0.2 * y + 0.3 * s1 * cxn;
+                b[1] = -0.2 * x - 0.3 * s2 * syn;
+                *c = -(1.0 + x * x);
+            } else {
+                sig[0] = orc_lut_eval(pb, 0, x, y); sig[1] = 0.0; sig[2] = 0.0;
+                sig[3] = orc_lut_eval(pb, 1, x, y);
+                b[0] = orc_lut_eval(pb, 2, x, y);
+                b[1] = orc_lut_eval(pb, 3, x, y);
+                *c = orc_lut_eval(pb, 4, x, y);
+            }
+            break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Monte Carlo: EMFKACSolver::Simulate_OMP (Solvers/EMFKACSolver.hpp:108-216) */
+/* ------------------------------------------------------------------------ */
+typedef struct { double x, y, Y, Z, xi; double sig[4], b[2], c, f, grad[2]; } orc_state;
+
+static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
+    double g, u;
+    orc_pde_eval(pb, s->x, s->y, s->sig, s->b, &s->c, &s->f, s->grad, &g, &u);
+}
+
+/* One control-variate Euler-Maruyama step (Integrators/FKACIntegrator.hpp:46-57):
+   all updates use the old X and Y; Dirichlet everywhere so psi = varphi = 0.
+   Coefficients at the old X are already in s. */
+static inline void orc_step(orc_state *s, double h, double sqrth, double n1, double n2) {
+    double i0 = sqrth * n1, i1 = sqrth * n2;
+    double v0 = -(s->sig[0] * s->grad[0] + s->sig[2] * s->grad[1]);
+    double v1 = -(s->sig[1] * s->grad[0] + s->sig[3] * s->grad[1]);
+    s->xi = s->xi + s->Y * (v0 * i0 + v1 * i1);
+    s->Z = s->Z + s->f * s->Y * h;
+    double Yold = s->Y;
+    s->Y = Yold + s->c * Yold * h;
+    double nx = s->x + (s->b[0] * h + (s->sig[0] * i0 + s->sig[1] * i1));
+    double ny = s->y + (s->b[1] * h + (s->sig[2] * i0 + s->sig[3] * i1));
+    s->x = nx; s->y = ny;
+}
+
+/* Inside() with the Gobet shift (Integrators/FKACIntegrator.hpp:168-186): returns 1 while the
+   path continues, i.e. dist < beta * |N^T sigma(X)| * sqrt(h).  Evaluates the coefficients at
+   the new X (sigma is needed here, the rest by the next step). */
+static inline int orc_inside(const orc_problem *pb, orc_state *s, const double rect[4], double beta,
+                             double sqrth, double *ex, double *ey) {
+    double nx, ny;
+    double d = orc_rect_dist(rect, s->x, s->y, ex, ey, &nx, &ny);
+    orc_coeffs(pb, s);
+    double a = nx * s->sig[0] + ny * s->sig[2];
+    double b = nx * s->sig[1] + ny * s->sig[3];
+    double nrm = sqrt(a * a + b * b);
+    return d < beta * nrm * sqrth;
+}
+
+void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y0, const double rect[4],
+                  uint32_t knot, int64_t path0, int64_t npaths,
+                  double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
+                  double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2) {
+    const double h = run->h, sqrth = sqrt(h);
+    const int64_t cap = run->max_steps > 0 ? run->max_steps : INT64_MAX;
+#ifdef _OPENMP
+    int nthr = run->threads > 0 ? run->threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthr)
+#endif
+    for (int64_t n = 0; n < npaths; ++n) {
+        uint32_t path = (uint32_t)(path0 + n);
+        orc_state s;
+        s.x = x0; s.y = y0; s.Y = 1.0; s.Z = 0.0; s.xi = 0.0;
+        orc_coeffs(pb, &s);
+        int64_t st = 0;
+        double ex = x0, ey = y0;
+        int in;
+        do {
+            double n1, n2;
+            orc_normal_pair(run, knot, path, st, &n1, &n2);
+            orc_step(&s, h, sqrth, n1, n2);
+            ++st;
+            in = orc_inside(pb, &s, rect, run->beta, sqrth, &ex, &ey);
+        } while (in && st < cap);
+        if (xt) xt[n] = ex;
+        if (yt) yt[n] = ey;
+        if (Y) Y[n] = s.Y;
+        if (Z) Z[n] = s.Z;
+        if (xi) xi[n] = s.xi;
+        if (steps) steps[n] = st;
+        if (run->phase2) {
+            /* sublinear continuation to the true exit (Solvers/EMFKACSolver.hpp:194-211) */
+            int64_t st2 = 0;
+            do {
+                double n1, n2;
+                orc_normal_pair(run, knot, path, st + st2, &n1, &n2);
+                orc_step(&s, h, sqrth, n1, n2);
+                ++st2;
+                in = orc_inside(pb, &s, rect, 0.0, sqrth, &ex, &ey);
+            } while (in && st2 < cap);
+            if (xt2) xt2[n] = ex;
+            if (yt2) yt2[n] = ey;
+            if (Y2) Y2[n] = s.Y;
+            if (Z2) Z2[n] = s.Z;
+            if (xi2) xi2[n] = s.xi;
+            if (steps2) steps2[n] = st2;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reductions                                                                */
+/* ------------------------------------------------------------------------ */
+static inline int orc_same(double a, double b) { return fabs(a - b) < 1e-8; } /* Meshes/stencil.cpp:186-189 */
+
+/* Exit side of a recorded exit point: 4 global boundary, 1 S, 0 N, 3 W, 2 E, -1 none.
+   Global test of Reduce_Analytic (Solvers/EMFKACSolver.hpp:559) then the S,N,W,E priority of
+   Stencil::G_update (Meshes/stencil.cpp:244-307). Side numbering N=0,S=1,E=2,W=3. */
+static inline int orc_classify(const double rect[4], const double glob[4], double x, double y) {
+    double ex, ey, nx, ny;
+    if (orc_rect_dist(glob, x, y, &ex, &ey, &nx, &ny) >= -1e-8) return 4;
+    if (orc_same(y, rect[1])) return 1;
+    if (orc_same(y, rect[3])) return 0;
+    if (orc_same(x, rect[0])) return 3;
+    if (orc_same(x, rect[2])) return 2;
+    return -1;
+}
+
+/* Gaussian RBF exp(-|p - X|^2 / c^2) with the reference's sqrt-then-square (BVPs/BVP.hpp:84-87) */
+static inline double orc_rbf(double px, double py, double x, double y, double c) {
+    double dx = px - x, dy = py - y;
+    double r = sqrt(dx * dx + dy * dy);
+    return orc_exp(-(r * r) / (c * c));
+}
+
+/* CSR row of a stencil: sorted unique columns; for each side entry its column slot. */
+static int orc_row_pattern(const int msz[4], const int *idx, int *cols, int *slot) {
+    int tot = msz[0] + msz[1] + msz[2] + msz[3];
+    int n = 0;
+    for (int e = 0; e < tot; ++e) {
+        int v = idx[e], found = 0;
+        for (int k = 0; k < n; ++k) if (cols[k] == v) { found = 1; break; }
+        if (!found) cols[n++] = v;
+    }
+    /* insertion sort */
+    for (int a = 1; a < n; ++a) {
+        int v = cols[a], b = a - 1;
+        while (b >= 0 && cols[b] > v) { cols[b + 1] = cols[b]; --b; }
+        cols[b + 1] = v;
+    }
+    for (int e = 0; e < tot; ++e)
+        for (int k = 0; k < n; ++k) if (cols[k] == idx[e]) { slot[e] = k; break; }
+    return n;
+}
+
+int orc_reduce_faithful(const orc_problem *pb, int64_t npaths, const double *xt, const double *yt,
+                        const double *Y, const double *Z, const double *xi,
+                        const double rect[4], const double glob[4], const int msz[4],
+                        const int *idx, const double *px, const double *py, const double cshape[4],
+                        const double *ipsi, int quirk,
+                        int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6]) {
+    int off[5] = {0, msz[0], msz[0] + msz[1], msz[0] + msz[1] + msz[2], msz[0] + msz[1] + msz[2] + msz[3]};
+    int64_t ioff[5];
+    ioff[0] = 0;
+    for (int s = 0; s < 4; ++s) ioff[s + 1] = ioff[s] + (int64_t)msz[s] * msz[s];
+    int tot = off[4];
+    double *Gs = (double *)calloc((size_t)tot + 1, sizeof(double));
+    double *GGs = (double *)calloc((size_t)tot + 1, sizeof(double));
+    double *phi = (double *)malloc(sizeof(double) * ((size_t)tot + 1));
+    for (int k = 0; k < 6; ++k) cnt[k] = 0;
+    double b = 0.0, bb = 0.0;
+    for (int64_t n = 0; n < npaths; ++n) {
+        int side = orc_classify(rect, glob, xt[n], yt[n]);
+        if (side == 4) {
+            double sig[4], bv[2], c, f, grad[2], g, u;
+            orc_pde_eval(pb, xt[n], yt[n], sig, bv, &c, &f, grad, &g, &u);
+            double sc = Z[n] + Y[n] * g + xi[n];
+            b += sc; bb += sc * sc; cnt[4]++;
+            continue;
+        }
+        if (side < 0) { cnt[5]++; }
+        else {
+            cnt[side]++;
+            int m = msz[side];
+            const double *A = ipsi + ioff[side];
+            for (int j = 0; j < m; ++j)
+                phi[j] = orc_rbf(px[off[side] + j], py[off[side] + j], xt[n], yt[n], cshape[side]);
+            for (int i = 0; i < m; ++i) {
+                double H = 0.0;
+                for (int j = 0; j < m; ++j) H += A[(int64_t)j * m + i] * phi[j];
+                Gs[off[side] + i] += -H * Y[n];
+                GGs[off[side] + i] += (H * Y[n]) * (H * Y[n]);
+            }
+        }
+        double sc = Z[n] + xi[n];
+        b += sc; bb += sc * sc;
+    }
+    int *slot = (int *)malloc(sizeof(int) * ((size_t)tot + 1));
+    int nnz = orc_row_pattern(msz, idx, cols, slot);
+    double *gg = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    for (int k = 0; k < nnz; ++k) G[k] = 0.0;
+    double norm = 1.0 / ((double)npaths * 1.0);
+    /* G_return_withrep order: N, S, E, W (Meshes/stencil.cpp:373-396) */
+    for (int e = 0; e < tot; ++e) {
+        if (quirk) { G[slot[e]] = Gs[e] * norm; gg[slot[e]] = GGs[e] * norm; }
+        else { G[slot[e]] += Gs[e] * norm; gg[slot[e]] += GGs[e] * norm; }
+    }
+    for (int k = 0; k < nnz; ++k) Gvar[k] = gg[k] - G[k] * G[k];
+    *B = b / (double)npaths;
+    *BB = bb / (double)npaths;
+    free(Gs); free(GGs); free(phi); free(slot); free(gg);
+    return nnz;
+}
+
+int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, const double *xt, const double *yt,
+                       const double *Y, const double *Z, const double *xi,
+                       const double rect[4], const double glob[4], const int msz[4],
+                       const int *idx, const double *px, const double *py, const double cshape[4],
+                       const double *ipsi, int quirk,
+                       int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6]) {
+    int off[5] = {0, msz[0], msz[0] + msz[1], msz[0] + msz[1] + msz[2], msz[0] + msz[1] + msz[2] + msz[3]};
+    int64_t ioff[5];
+    ioff[0] = 0;
+    for (int s = 0; s < 4; ++s) ioff[s + 1] = ioff[s] + (int64_t)msz[s] * msz[s];
+    int tot = off[4];
+    int *slot = (int *)malloc(sizeof(int) * ((size_t)tot + 1));
+    int nnz = orc_row_pattern(msz, idx, cols, slot);
+    /* in quirk mode only the last side listing a column contributes to it */
+    int *use = (int *)malloc(sizeof(int) * ((size_t)tot + 1));
+    for (int e = 0; e < tot; ++e) {
+        use[e] = 1;
+        if (quirk) for (int e2 = e + 1; e2 < tot; ++e2) if (slot[e2] == slot[e]) { use[e] = 0; break; }
+    }
+    double *Phi = (double *)calloc((size_t)tot + 1, sizeof(double));
+    double *Gb = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    double *sum = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    double *sq = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    for (int k = 0; k < 6; ++k) cnt[k] = 0;
+    double b = 0.0, bb = 0.0;
+    int64_t nb = 0;
+    for (int64_t p0 = 0; p0 < npaths; p0 += S) {
+        int64_t p1 = p0 + S < npaths ? p0 + S : npaths;
+        memset(Phi, 0, sizeof(double) * (size_t)tot);
+        for (int64_t n = p0; n < p1; ++n) {
+            int side = orc_classify(rect, glob, xt[n], yt[n]);
+            if (side == 4) {
+                double sig[4], bv[2], c, f, grad[2], g, u;
+                orc_pde_eval(pb, xt[n], yt[n], sig, bv, &c, &f, grad, &g, &u);
+                double sc = Z[n] + Y[n] * g + xi[n];
+                b += sc; bb += sc * sc; cnt[4]++;
+                continue;
+            }
+            if (side < 0) cnt[5]++;
+            else {
+                cnt[side]++;
+                for (int j = 0; j < msz[side]; ++j)
+                    Phi[off[side] + j] += Y[n] * orc_rbf(px[off[side] + j], py[off[side] + j], xt[n], yt[n], cshape[side]);
+            }
+            double sc = Z[n] + xi[n];
+            b += sc; bb += sc * sc;
+        }
+        memset(Gb, 0, sizeof(double) * (size_t)nnz);
+        for (int s = 0; s < 4; ++s) {
+            int m = msz[s];
+            const double *A = ipsi + ioff[s];
+            for (int i = 0; i < m; ++i) {
+                int e = off[s] + i;
+                if (!use[e]) continue;
+                double H = 0.0;
+                for (int j = 0; j < m; ++j) H += A[(int64_t)j * m + i] * Phi[off[s] + j];
+                Gb[slot[e]] += -H;
+            }
+        }
+        double nbeta = (double)(p1 - p0);
+        for (int k = 0; k < nnz; ++k) { sum[k] += Gb[k]; sq[k] += Gb[k] * Gb[k] / nbeta; }
+        nb++;
+    }
+    double N = (double)npaths;
+    for (int k = 0; k < nnz; ++k) {
+        G[k] = sum[k] / N;
+        Gvar[k] = nb > 1 ? (sq[k] - N * G[k] * G[k]) / (double)(nb - 1) : 0.0;
+    }
+    *B = b / N;
+    *BB = bb / N;
+    free(slot); free(use); free(Phi); free(Gb); free(sum); free(sq);
+    return nnz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Psi inverse by full-pivot LU (Meshes/stencil.cpp:115-184)                 */
+/* ------------------------------------------------------------------------ */
+int orc_stencil_ipsi(int m, const double *px, const double *py, double c, double *ipsi, double *cond) {
+    double *A = (double *)malloc(sizeof(double) * (size_t)m * m);
+    double *LU = (double *)malloc(sizeof(double) * (size_t)m * m);
+    int *pr = (int *)malloc(sizeof(int) * (size_t)m), *pc = (int *)malloc(sizeof(int) * (size_t)m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A[(int64_t)i * m + j] = orc_rbf(px[i], py[i], px[j], py[j], c);
+    memcpy(LU, A, sizeof(double) * (size_t)m * m);
+    for (int i = 0; i < m; ++i) { pr[i] = i; pc[i] = i; }
+    double maxpiv = 0.0;
+    int rank = m;
+    for (int k = 0; k < m; ++k) {
+        int bi = k, bj = k; double best = -1.0;
+        for (int i = k; i < m; ++i)
+            for (int j = k; j < m; ++j) { double v = fabs(LU[(int64_t)i * m + j]); if (v > best) { best = v; bi = i; bj = j; } }
+        if (k == 0) maxpiv = best;
+        if (best <= maxpiv * 2.220446049250313e-16 * m) { rank = k; break; }
+        if (bi != k) { for (int j = 0; j < m; ++j) { double t = LU[(int64_t)k * m + j]; LU[(int64_t)k * m + j] = LU[(int64_t)bi * m + j]; LU[(int64_t)bi * m + j] = t; } int t = pr[k]; pr[k] = pr[bi]; pr[bi] = t; }
+        if (bj != k) { for (int i = 0; i < m; ++i) { double t = LU[(int64_t)i * m + k]; LU[(int64_t)i * m + k] = LU[(int64_t)i * m + bj]; LU[(int64_t)i * m + bj] = t; } int t = pc[k]; pc[k] = pc[bj]; pc[bj] = t; }
+        double piv = LU[(int64_t)k * m + k];
+        for (int i = k + 1; i < m; ++i) {
+            double l = LU[(int64_t)i * m + k] / piv;
+            LU[(int64_t)i * m + k] = l;
+            for (int j = k + 1; j < m; ++j) LU[(int64_t)i * m + j] -= l * LU[(int64_t)k * m + j];
+        }
+    }
+    int status = 0;
+    if (rank < m) status = 1;
+    else {
+        /* P A Q = L U  =>  A^-1 = Q U^-1 L^-1 P */
+        double *col = (double *)malloc(sizeof(double) * (size_t)m);
+        for (int e = 0; e < m; ++e) {
+            /* solve A z = e_e : L U (Q^T z) = P e_e */
+            for (int i = 0; i < m; ++i) col[i] = (pr[i] == e) ? 1.0 : 0.0;
+            for (int i = 0; i < m; ++i) { double s = col[i]; for (int j = 0; j < i; ++j) s -= LU[(int64_t)i * m + j] * col[j]; col[i] = s; }
+            for (int i = m - 1; i >= 0; --i) { double s = col[i]; for (int j = i + 1; j < m; ++j) s -= LU[(int64_t)i * m + j] * col[j]; col[i] = s / LU[(int64_t)i * m + i]; }
+            for (int i = 0; i < m; ++i) ipsi[(int64_t)pc[i] * m + e] = col[i];
+        }
+        free(col);
+    }
+    double na = 0.0, ni = 0.0;
+    for (int64_t k = 0; k < (int64_t)m * m; ++k) { na += A[k] * A[k]; if (!status) ni += ipsi[k] * ipsi[k]; }
+    *cond = status ? INFINITY : sqrt(na) * sqrt(ni);
+    free(A); free(LU); free(pr); free(pc);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BiCGSTAB (van der Vorst 1992) on (I + G) with the diagonal folded in:    */
+/* the caller passes the full matrix rows.  Zero initial guess, stop when    */
+/* |r| <= tol |b| or maxit iterations (PDDAlgorithms/PDDSparseGM.cpp:3228-3232). */
+/* ------------------------------------------------------------------------ */
+static void orc_spmv(int n, const int64_t *rp, const int *ci, const double *v, const double *x, double *y) {
+    for (int i = 0; i < n; ++i) { double s = 0.0; for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]]; y[i] = s; }
+}
+static double orc_dot(int n, const double *a, const double *b) { double s = 0.0; for (int i = 0; i < n; ++i) s += a[i] * b[i]; return s; }
+
+int orc_bicgstab(int n, const int64_t *rp, const int *ci, const double *v, const double *b,
+                 double *x, double tol, int maxit, double *relres) {
+    double *r = calloc((size_t)n, 8), *r0 = calloc((size_t)n, 8), *p = calloc((size_t)n, 8), *vv = calloc((size_t)n, 8);
+    double *s = calloc((size_t)n, 8), *t = calloc((size_t)n, 8);
+    for (int i = 0; i < n; ++i) { x[i] = 0.0; r[i] = b[i]; r0[i] = b[i]; }
+    double bn2 = orc_dot(n, b, b);
+    if (bn2 == 0.0) { *relres = 0.0; free(r); free(r0); free(p); free(vv); free(s); free(t); return 0; }
+    double rho = 1.0, alpha = 1.0, w = 1.0, r0n2 = bn2;
+    int it = 0;
+    double tol2 = tol * tol * bn2;
+    while (orc_dot(n, r, r) > tol2 && it < maxit) {
+        double rho_old = rho;
+        rho = orc_dot(n, r0, r);
+        if (fabs(rho) < 1e-30 * r0n2) { /* breakdown: restart with the current residual */
+            orc_spmv(n, rp, ci, v, x, t);
+            for (int i = 0; i < n; ++i) { r[i] = b[i] - t[i]; r0[i] = r[i]; }
+            rho = r0n2 = orc_dot(n, r, r);
+        }
+        double beta = (rho / rho_old) * (alpha / w);
+        for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - w * vv[i]);
+        orc_spmv(n, rp, ci, v, p, vv);
+        alpha = rho / orc_dot(n, r0, vv);
+        for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * vv[i];
+        orc_spmv(n, rp, ci, v, s, t);
+        double tt = orc_dot(n, t, t);
+        w = tt > 0.0 ? orc_dot(n, t, s) / tt : 0.0;
+        for (int i = 0; i < n; ++i) { x[i] += alpha * p[i] + w * s[i]; r[i] = s[i] - w * t[i]; }
+        ++it;
+    }
+    *relres = sqrt(orc_dot(n, r, r) / bn2);
+    free(r); free(r0); free(p); free(vv); free(s); free(t);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LUT prefilter: interpolating cubic B-spline coefficients                  */
+/* ------------------------------------------------------------------------ */
+/* 1-D: given s[0..n-1] (stride ss) find c[-1..n] (n+2 values, stride cs) with
+   (c[k-1] + 4 c[k] + c[k+1]) / 6 = s[k], and end conditions matching the
+   spline's second difference to the data's, extrapolated to the end node. */
+static void orc_prefilter_1d(int n, const double *s, int ss, double *c, int cs, double *work) {
+    double rL = 2.0 * (s[0] - 2.0 * s[ss] + s[2 * ss]) - (s[ss] - 2.0 * s[2 * ss] + s[3 * ss]);
+    double rR = 2.0 * (s[(n - 1) * ss] - 2.0 * s[(n - 2) * ss] + s[(n - 3) * ss])
+                - (s[(n - 2) * ss] - 2.0 * s[(n - 3) * ss] + s[(n - 4) * ss]);
+    double c0 = s[0] - rL / 6.0, cn1 = s[(n - 1) * ss] - rR / 6.0;
+    /* unknowns c[1..n-2]: c[k-1] + 4 c[k] + c[k+1] = 6 s[k] */
+    int m = n - 2;
+    double *cp = work, *dp = work + m;
+    for (int k = 0; k < m; ++k) {
+        double rhs = 6.0 * s[(k + 1) * ss];
+        if (k == 0) rhs -= c0;
+        if (k == m - 1) rhs -= cn1;
+        double denom = 4.0 - (k > 0 ? cp[k - 1] : 0.0);
+        cp[k] = 1.0 / denom;
+        dp[k] = (rhs - (k > 0 ? dp[k - 1] : 0.0)) / denom;
+    }
+    double *cc = (double *)malloc(sizeof(double) * (size_t)n);
+    cc[0] = c0; cc[n - 1] = cn1;
+    for (int k = m - 1; k >= 0; --k) cc[k + 1] = dp[k] - cp[k] * (k + 1 < m ? cc[k + 2] : 0.0);
+    c[0] = 2.0 * cc[0] - cc[1] + rL;
+    for (int k = 0; k < n; ++k) c[(k + 1) * cs] = cc[k];
+    c[(n + 1) * cs] = 2.0 * cc[n - 1] - cc[n - 2] + rR;
+    free(cc);
+}
+
+void orc_lut_prefilter(int nx, int ny, int nf, const double *samples, float *coef) {
+    int NX = nx + 2, NY = ny + 2;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)NX * ny);
+    double *out = (double *)malloc(sizeof(double) * (size_t)NX * NY);
+    double *work = (double *)malloc(sizeof(double) * 2 * (size_t)(nx > ny ? nx : ny));
+    for (int f = 0; f < nf; ++f) {
+        double *row = (double *)malloc(sizeof(double) * (size_t)nx);
+        for (int j = 0; j < ny; ++j) {
+            for (int i = 0; i < nx; ++i) row[i] = samples[((int64_t)j * nx + i) * nf + f];
+            orc_prefilter_1d(nx, row, 1, tmp + (int64_t)j * NX, 1, work);
+        }
+        free(row);
+        double *colv = (double *)malloc(sizeof(double) * (size_t)ny);
+        double *colc = (double *)malloc(sizeof(double) * (size_t)NY);
+        for (int i = 0; i < NX; ++i) {
+            for (int j = 0; j < ny; ++j) colv[j] = tmp[(int64_t)j * NX + i];
+            orc_prefilter_1d(ny, colv, 1, colc, 1, work);
+            for (int j = 0; j < NY; ++j) out[(int64_t)j * NX + i] = colc[j];
+        }
+        free(colv); free(colc);
+        for (int64_t k = 0; k < (int64_t)NX * NY; ++k) coef[k * nf + f] = (float)out[k];
+    }
+    free(tmp); free(out); free(work);
+}

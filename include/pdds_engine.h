@@ -1,0 +1,172 @@
+/*
+ * pdds_engine.h -- C ABI of the MI355X PDDSparse path engine (libpdds.so).
+ *
+ * Plain C types only.  Every entry point returns a pdds_status; no entry point
+ * prints or exits.  The caller owns every host buffer; the engine owns device
+ * memory.  One engine is bound to one HIP device.  Calls on one engine are
+ * serialised by the engine; separate engines may be used from separate threads.
+ * Results are a function of (seed, stream, knot id, path index) only, never of
+ * the launch shape or of what else runs on the device.
+ *
+ * What each group of entry points replaces in the reference (file:line):
+ *   pdds_problem            struct bvp function-pointer plugin (BVPs/BVP.hpp:89-137),
+ *                           compile-time device twin (BVPs/BVP.cuh:44-77, BVPs/initBVPdev.cuh:6-16)
+ *   pdds_lut_*              Init_tex_LUT / Init_LUT (BVPs/LUT2.cuh:11-92) + cubicTex2DSimple
+ *                           (Integrators/cubicTex2D.cu:64-87)
+ *   pdds_mesh_*             PDDSparseGM::Fullfill_interfaces (PDDAlgorithms/PDDSparseGM.cpp:346-689),
+ *                           Labels_Stencil / Compute_Stencil (:1330-1433, :1845-2041),
+ *                           Stencil::Init / Compute_ipsi (Meshes/stencil.cpp:11-85, :115-243)
+ *   pdds_paths              MCinCUDA (Solvers/MCinCUDA.cu:441-480, declared Solvers/EMFKACSolver.hpp:10-12):
+ *                           per-path exit records, phase 1 and optional phase 2
+ *   pdds_solve_knot         EMFKACSolver::Solve_OMP_Analytic with a stencil (Solvers/EMFKACSolver.hpp:711-729):
+ *                           one knot's G row, G variances, B and B second moment
+ *   pdds_point_eval         EMFKACSolver::Solve_OMP_Analytic without a stencil (Solvers/EMFKACSolver.hpp:687-692):
+ *                           Feynman-Kac estimate of u at one point
+ *   pdds_assemble           the worker loop of PDDSparseGM::Solve (PDDAlgorithms/PDDSparseGM.cpp:795-848):
+ *                           every knot of a mesh in one persistent launch, straight into CSR
+ *   pdds_solve              Compute_Solution's BiCGSTAB on (I+G)u = B (PDDAlgorithms/PDDSparseGM.cpp:3168-3246)
+ *   pdds_server_*           no reference counterpart: continuous-batching request loop
+ */
+#ifndef PDDS_ENGINE_H
+#define PDDS_ENGINE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pdds_status;
+#define PDDS_OK 0
+#define PDDS_ERR_INVALID 1      /* bad argument or descriptor */
+#define PDDS_ERR_NOMEM 2        /* device or host allocation failed */
+#define PDDS_ERR_DEVICE 3       /* HIP runtime error */
+#define PDDS_ERR_UNSUPPORTED 4  /* valid request the engine does not implement */
+#define PDDS_ERR_SINGULAR 5     /* an RBF interpolation matrix Psi is not invertible */
+#define PDDS_ERR_NOT_READY 6    /* serving: ticket not finished yet */
+#define PDDS_ERR_FULL 7         /* serving: queue full / request exceeds the HBM budget */
+
+typedef struct pdds_engine pdds_engine;
+typedef struct pdds_mesh pdds_mesh;
+typedef struct pdds_lut pdds_lut;
+
+/* ---- lifecycle ---- */
+pdds_status pdds_engine_create(int device, pdds_engine **out);
+void pdds_engine_destroy(pdds_engine *e);
+const char *pdds_last_error(const pdds_engine *e);
+const char *pdds_status_string(pdds_status s);
+/* number of compute units and the device name, for launch sizing and reports */
+pdds_status pdds_engine_info(const pdds_engine *e, int32_t *n_cu, char *name, int32_t name_len);
+
+/* ---- problem descriptor (reference convention 1/2 s s^T : D2u + b.Du + c u + f = 0, c <= 0) ---- */
+#define PDDS_PDE_MONEGROS 0      /* BVPs/EDPs/Monegros_Poisson_CPP.hpp:12-88 */
+#define PDDS_PDE_SINSIN 1        /* u = sin(pi x) sin(pi y) + x^2 + y^2, sigma = sqrt2 I, c = 0 */
+#define PDDS_PDE_VARCOEF 2       /* div form 1/2 div(s s^T grad u) + b.grad u - c u = f, u = e^x cos y */
+#define PDDS_PDE_VARCOEF_LUT 3   /* PDE 2 with sigma11, sigma22, b1, b2, c_ref from a 5-field LUT */
+#define PDDS_PDE_HELMHOLTZ 4     /* u as PDE 1, c = -k2 */
+typedef struct {
+    int32_t pde;
+    double k2;              /* PDDS_PDE_HELMHOLTZ only */
+    const pdds_lut *lut;    /* PDDS_PDE_VARCOEF_LUT only */
+} pdds_problem;
+
+/* ---- run parameters ---- */
+#define PDDS_FP64 0             /* FP64 path state; bit-exact with the FP64 oracle */
+#define PDDS_FP32 1             /* FP32 path state, Kahan functionals, FP64 cross-path sums */
+typedef struct {
+    double h;               /* Euler-Maruyama step */
+    double beta;            /* Gobet constant, reference -0.5826 (Solvers/EMFKACSolver.hpp:186) */
+    uint32_t seed;          /* Philox key word 0 */
+    uint32_t stream;        /* Philox key word 1: request id, 0 for plain assembly */
+    int32_t precision;      /* PDDS_FP64 or PDDS_FP32 */
+    int32_t quirk_last_side;/* 1: corner knots keep the last side's value (Meshes/stencil.cpp:376-395) */
+    int64_t max_steps;      /* per-path step cap (0: 2^26) */
+    int32_t sub_batch;      /* paths per variance batch S (0: 32) */
+    int32_t item_paths;     /* paths per work item P, multiple of S (0: chosen from the path count) */
+    int32_t wg_size;        /* work-group size (0: 256) */
+    int32_t grid_wgs;       /* persistent work-groups (0: 2 per CU) */
+    int32_t lut_lds;        /* 1: stage LUT tiles in LDS when they fit (default 1) */
+    int32_t plain;          /* 1: plain launch -- one path per lane, one work-group per item of wg_size
+                               paths, no lane recycling, no persistent queue (the baseline GPU form) */
+} pdds_run;
+void pdds_run_defaults(pdds_run *r);
+
+/* ---- coefficient LUTs (bicubic B-spline, FP32 coefficients, device resident) ----
+   samples: [ny][nx][nf] FP64 on the box [x0,x1]x[y0,y1].
+   mode 0: node-aligned samples, interpolating spline (prefiltered coefficients);
+   mode 1: texel-centred samples used raw as coefficients (cubicTex2DSimple semantics). */
+pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, const double *samples,
+                            double x0, double y0, double x1, double y1, int32_t mode, pdds_lut **out);
+void pdds_lut_destroy(pdds_lut *l);
+/* evaluate field f of the LUT at n points on the device (FP64 arithmetic) */
+pdds_status pdds_lut_eval(pdds_engine *e, const pdds_lut *l, int32_t field, int64_t n, const double *x,
+                          const double *y, double *out);
+
+/* ---- mesh: interfaces, knots, stencils, CSR pattern, Psi^-1 ---- */
+pdds_status pdds_mesh_create(double sw_x, double sw_y, double ne_x, double ne_y, int32_t n_sub,
+                             int32_t n_node, const double fac[4], pdds_mesh **out);
+void pdds_mesh_destroy(pdds_mesh *m);
+int32_t pdds_mesh_n_knots(const pdds_mesh *m);
+int32_t pdds_mesh_n_stencils(const pdds_mesh *m);      /* distinct stencils */
+/* start position (dispatch order), reported position (Node_Position), boundary flag, stencil id */
+pdds_status pdds_mesh_knots(const pdds_mesh *m, double *x0, double *y0, double *px, double *py,
+                            int32_t *on_boundary, int32_t *stencil_id);
+/* stencil of knot k: rect[4], side sizes msz[4] and kinds[4] (order N,S,E,W); side lists
+   concatenated into idx/px/py (capacity cap); *n_entries = sum msz */
+pdds_status pdds_mesh_stencil(const pdds_mesh *m, int32_t knot, double rect[4], int32_t msz[4],
+                              int32_t kind[4], int32_t cap, int32_t *idx, double *px, double *py,
+                              int32_t *n_entries);
+/* condition numbers |Psi|_F |Psi^-1|_F per side of knot k (0 for absent sides) */
+pdds_status pdds_mesh_cond(const pdds_mesh *m, int32_t knot, double cond[4]);
+/* Psi^-1 of side `side` (0 N, 1 S, 2 E, 3 W) of knot k, row-major m x m into out (capacity cap) */
+pdds_status pdds_mesh_ipsi(const pdds_mesh *m, int32_t knot, int32_t side, int64_t cap, double *out, int32_t *msz);
+/* CSR pattern of G without the identity: row_ptr[n_knots+1]; cols may be NULL to query nnz */
+pdds_status pdds_mesh_csr(const pdds_mesh *m, int64_t *row_ptr, int32_t *cols, int64_t *nnz);
+
+/* ---- counters ---- */
+typedef struct {
+    int64_t path_steps;     /* phase-1 Euler-Maruyama steps over all paths */
+    int64_t paths;
+    int64_t exits[6];       /* N, S, E, W, global boundary, unclassified */
+    int64_t capped;         /* paths stopped by max_steps */
+    double kernel_ms;       /* device time of the path kernel (HIP events) */
+    double reduce_ms;       /* device time of the finalize kernel */
+    double total_ms;        /* host wall time of the call */
+    int64_t work_items;
+    int32_t grid_wgs, wg_size, item_paths, sub_batch;
+} pdds_counters;
+
+/* ---- per-path records (MCinCUDA shape) ----
+   Paths path0..path0+n-1 of knot id `knot` start at (x0,y0) and stop on rect[4].
+   Outputs (any may be NULL): exit projection xt/yt, Y, Z, xi, steps; phase-2 outputs when
+   phase2 != 0 (continuation to the true exit with beta = 0). */
+pdds_status pdds_paths(pdds_engine *e, const pdds_problem *p, const pdds_run *r, double x0, double y0,
+                       const double rect[4], uint32_t knot, int64_t path0, int64_t n, int32_t phase2,
+                       double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
+                       double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2,
+                       pdds_counters *ctr);
+
+/* ---- one knot: G row (mesh CSR order), G variance, B, BB ---- */
+pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                            int32_t knot, int64_t n_paths, double *G, double *G_var, double *B, double *BB,
+                            pdds_counters *ctr);
+
+/* ---- every knot of a mesh: G values in the mesh CSR pattern, per-entry per-path variance,
+   B and B second moment per knot.  paths_per_knot may be NULL (n_paths each). ---- */
+pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                          const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var,
+                          double *B, double *BB, pdds_counters *ctr);
+
+/* ---- Feynman-Kac estimate of u at n points of the global rectangle `glob` ---- */
+pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
+                            int32_t n, const double *x, const double *y, const int64_t *n_paths,
+                            double *u, double *u_var, pdds_counters *ctr);
+
+/* ---- (I + G) u = B by BiCGSTAB on the device, zero initial guess ----
+   rows given without the identity (the mesh CSR pattern). Returns iterations in *iters. */
+pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_ptr, const int32_t *cols,
+                       const double *G, const double *B, double *u, double tol, int32_t maxit,
+                       int32_t *iters, double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

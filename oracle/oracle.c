@@ -417,8 +417,8 @@ int orc_reduce_faithful(const orc_problem *pb, int64_t npaths, const double *xt,
     return nnz;
 }
 
-int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, const double *xt, const double *yt,
-                       const double *Y, const double *Z, const double *xi,
+int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
+                       const double *yt, const double *Y, const double *Z, const double *xi,
                        const double rect[4], const double glob[4], const int msz[4],
                        const int *idx, const double *px, const double *py, const double cshape[4],
                        const double *ipsi, int quirk,
@@ -438,47 +438,65 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, const d
     }
     double *Phi = (double *)calloc((size_t)tot + 1, sizeof(double));
     double *Gb = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    double *isum = (double *)calloc((size_t)nnz + 1, sizeof(double));
+    double *isq = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *sum = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *sq = (double *)calloc((size_t)nnz + 1, sizeof(double));
     for (int k = 0; k < 6; ++k) cnt[k] = 0;
     double b = 0.0, bb = 0.0;
     int64_t nb = 0;
-    for (int64_t p0 = 0; p0 < npaths; p0 += S) {
-        int64_t p1 = p0 + S < npaths ? p0 + S : npaths;
-        memset(Phi, 0, sizeof(double) * (size_t)tot);
-        for (int64_t n = p0; n < p1; ++n) {
-            int side = orc_classify(rect, glob, xt[n], yt[n]);
-            if (side == 4) {
-                double sig[4], bv[2], c, f, grad[2], g, u;
-                orc_pde_eval(pb, xt[n], yt[n], sig, bv, &c, &f, grad, &g, &u);
-                double sc = Z[n] + Y[n] * g + xi[n];
-                b += sc; bb += sc * sc; cnt[4]++;
-                continue;
+    /* two-level order: sub-batches of S paths summed into items of P paths, items summed into the knot */
+    for (int64_t w0 = 0; w0 < npaths; w0 += P) {
+        int64_t w1 = w0 + P < npaths ? w0 + P : npaths;
+        double ib = 0.0, ibb = 0.0;
+        memset(isum, 0, sizeof(double) * (size_t)nnz);
+        memset(isq, 0, sizeof(double) * (size_t)nnz);
+        for (int64_t p0 = w0; p0 < w1; p0 += S) {
+            int64_t p1 = p0 + S < w1 ? p0 + S : w1;
+            memset(Phi, 0, sizeof(double) * (size_t)tot);
+            double sb = 0.0, sbb = 0.0;
+            for (int64_t n = p0; n < p1; ++n) {
+                int side = orc_classify(rect, glob, xt[n], yt[n]);
+                double sc;
+                if (side == 4) {
+                    double sig[4], bv[2], c, f, grad[2], g, u;
+                    orc_pde_eval(pb, xt[n], yt[n], sig, bv, &c, &f, grad, &g, &u);
+                    sc = Z[n] + Y[n] * g + xi[n];
+                    cnt[4]++;
+                } else {
+                    if (side < 0) cnt[5]++;
+                    else {
+                        cnt[side]++;
+                        for (int j = 0; j < msz[side]; ++j)
+                            Phi[off[side] + j] = Phi[off[side] + j] +
+                                Y[n] * orc_rbf(px[off[side] + j], py[off[side] + j], xt[n], yt[n], cshape[side]);
+                    }
+                    sc = Z[n] + xi[n];
+                }
+                sb = sb + sc;
+                sbb = sbb + sc * sc;
             }
-            if (side < 0) cnt[5]++;
-            else {
-                cnt[side]++;
-                for (int j = 0; j < msz[side]; ++j)
-                    Phi[off[side] + j] += Y[n] * orc_rbf(px[off[side] + j], py[off[side] + j], xt[n], yt[n], cshape[side]);
+            memset(Gb, 0, sizeof(double) * (size_t)nnz);
+            for (int s = 0; s < 4; ++s) {
+                int m = msz[s];
+                const double *A = ipsi + ioff[s];
+                for (int i = 0; i < m; ++i) {
+                    int e = off[s] + i;
+                    if (!use[e]) continue;
+                    double H = 0.0;
+                    for (int j = 0; j < m; ++j) H = H + A[(int64_t)j * m + i] * Phi[off[s] + j];
+                    Gb[slot[e]] = Gb[slot[e]] + (-H);
+                }
             }
-            double sc = Z[n] + xi[n];
-            b += sc; bb += sc * sc;
+            double nbeta = (double)(p1 - p0);
+            for (int k = 0; k < nnz; ++k) { isum[k] = isum[k] + Gb[k]; isq[k] = isq[k] + (Gb[k] * Gb[k]) / nbeta; }
+            ib = ib + sb;
+            ibb = ibb + sbb;
+            nb++;
         }
-        memset(Gb, 0, sizeof(double) * (size_t)nnz);
-        for (int s = 0; s < 4; ++s) {
-            int m = msz[s];
-            const double *A = ipsi + ioff[s];
-            for (int i = 0; i < m; ++i) {
-                int e = off[s] + i;
-                if (!use[e]) continue;
-                double H = 0.0;
-                for (int j = 0; j < m; ++j) H += A[(int64_t)j * m + i] * Phi[off[s] + j];
-                Gb[slot[e]] += -H;
-            }
-        }
-        double nbeta = (double)(p1 - p0);
-        for (int k = 0; k < nnz; ++k) { sum[k] += Gb[k]; sq[k] += Gb[k] * Gb[k] / nbeta; }
-        nb++;
+        for (int k = 0; k < nnz; ++k) { sum[k] = sum[k] + isum[k]; sq[k] = sq[k] + isq[k]; }
+        b = b + ib;
+        bb = bb + ibb;
     }
     double N = (double)npaths;
     for (int k = 0; k < nnz; ++k) {
@@ -487,7 +505,7 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, const d
     }
     *B = b / N;
     *BB = bb / N;
-    free(slot); free(use); free(Phi); free(Gb); free(sum); free(sq);
+    free(slot); free(use); free(Phi); free(Gb); free(isum); free(isq); free(sum); free(sq);
     return nnz;
 }
 
@@ -541,12 +559,13 @@ int orc_stencil_ipsi(int m, const double *px, const double *py, double c, double
 }
 
 /* ------------------------------------------------------------------------ */
-/* BiCGSTAB (van der Vorst 1992) on (I + G) with the diagonal folded in:    */
-/* the caller passes the full matrix rows.  Zero initial guess, stop when    */
-/* |r| <= tol |b| or maxit iterations (PDDAlgorithms/PDDSparseGM.cpp:3228-3232). */
+/* BiCGSTAB (van der Vorst 1992) on (I + G): the caller passes the rows of G */
+/* without the identity (Compute_Solution adds it, PDDSparseGM.cpp:3187-3189). */
+/* Zero initial guess, stop when |r| <= tol |b| or maxit iterations          */
+/* (PDDAlgorithms/PDDSparseGM.cpp:3228-3232).                                */
 /* ------------------------------------------------------------------------ */
 static void orc_spmv(int n, const int64_t *rp, const int *ci, const double *v, const double *x, double *y) {
-    for (int i = 0; i < n; ++i) { double s = 0.0; for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]]; y[i] = s; }
+    for (int i = 0; i < n; ++i) { double s = x[i]; for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]]; y[i] = s; }
 }
 static double orc_dot(int n, const double *a, const double *b) { double s = 0.0; for (int i = 0; i < n; ++i) s += a[i] * b[i]; return s; }
 

@@ -106,8 +106,9 @@ int orc_reduce_faithful(const orc_problem *pb, int64_t npaths, const double *xt,
                         const double *ipsi, int quirk,
                         int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6]);
 /* Restructured estimator: per sub-batch of S paths, Phi = sum Y phi(X) per side, G_beta = -Psi^-T Phi;
-   G = sum G_beta / N; Gvar = (sum G_beta^2/n_beta - N G^2) / (nbatches - 1)  (batch means). */
-int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, const double *xt, const double *yt,
+   G = sum G_beta / N; Gvar = (sum G_beta^2/n_beta - N G^2) / (nbatches - 1)  (batch means).
+   Sums run sub-batch -> item of P paths -> knot, the order the GPU engine uses. */
+int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt, const double *yt,
                        const double *Y, const double *Z, const double *xi,
                        const double rect[4], const double glob[4], const int msz[4],
                        const int *idx, const double *px, const double *py, const double cshape[4],

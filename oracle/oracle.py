@@ -65,7 +65,7 @@ def lib():
         red = [C.POINTER(Problem), C.c_int64, _d, _d, _d, _d, _d, _d, _d, _i, _i, _d, _d, _d, _d, C.c_int,
                _i, _d, _d, _d, _d, _i64]
         L.orc_reduce_faithful.argtypes = red
-        L.orc_reduce_batched.argtypes = red[:2] + [C.c_int64] + red[2:]
+        L.orc_reduce_batched.argtypes = red[:2] + [C.c_int64, C.c_int64] + red[2:]
         L.orc_bicgstab.argtypes = [C.c_int, _i64, _i, _d, _d, _d, C.c_double, C.c_int, _d]
         L.orc_philox.argtypes = [_u32, _u32, _u32]
         L.orc_math_eval.argtypes = [C.c_int, _d, _d, C.c_int64]
@@ -198,7 +198,7 @@ def simulate(pb, run, x0, y0, rect, knot, path0, npaths, phase2_out=False):
     return o
 
 
-def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None):
+def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None):
     msz = np.ascontiguousarray(st["msz"], dtype=np.int32)
     idx = np.ascontiguousarray(st["idx"], dtype=np.int32)
     tot = int(msz.sum())
@@ -213,7 +213,7 @@ def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None):
     n = len(paths["Y"])
     args = [C.byref(pb), n]
     if S is not None:
-        args.append(S)
+        args += [S, P if P else n]
     args += [ptr(paths["xt"]), ptr(paths["yt"]), ptr(paths["Y"]), ptr(paths["Z"]), ptr(paths["xi"]),
              ptr(np.ascontiguousarray(st["rect"], dtype=np.float64)), ptr(np.ascontiguousarray(glob, dtype=np.float64)),
              ptr(msz, _i), ptr(idx, _i), ptr(np.ascontiguousarray(st["px"])), ptr(np.ascontiguousarray(st["py"])),
@@ -226,8 +226,9 @@ def reduce_faithful(pb, paths, st, glob, cshape, ipsi_list, quirk=1):
     return _reduce(lib().orc_reduce_faithful, pb, paths, st, glob, cshape, ipsi_list, quirk)
 
 
-def reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, quirk=1):
-    return _reduce(lib().orc_reduce_batched, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S)
+def reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, P=None, quirk=1):
+    """Batch-means estimator with the engine's summation order (sub-batches of S in items of P)."""
+    return _reduce(lib().orc_reduce_batched, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P)
 
 
 def bicgstab(rowptr, cols, vals, b, tol=1e-14, maxit=None):

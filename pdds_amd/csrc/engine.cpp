@@ -1,0 +1,787 @@
+// engine.cpp -- C ABI of the path engine (include/pdds_engine.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pdds_engine.h"
+#include "engine_types.h"
+#include "mesh.h"
+#include "solver.h"
+
+using namespace pdds;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct pdds_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    std::mutex mu;
+    int n_cu = 0;
+    std::string name;
+    std::vector<DevBuf> bufs;   // grow-only scratch, indexed by slot
+};
+
+struct pdds_lut {
+    pdds_engine *e = nullptr;
+    float *d = nullptr;
+    LutView view{};
+    int nx = 0, ny = 0, nf = 0, mode = 0;
+};
+
+struct pdds_mesh {
+    Mesh m;
+    pdds_mesh(double a, double b, double c, double d, int n, int k, const double f[4]) : m(a, b, c, d, n, k, f) {}
+};
+
+namespace {
+
+enum BufSlot {
+    B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_RECX, B_RECY,
+    B_RECYY, B_RECS, B_RECSIDE, B_COUNTER, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
+    B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
+};
+
+pdds_status fail(pdds_engine *e, pdds_status s, const std::string &msg) {
+    if (e) e->err = msg;
+    return s;
+}
+
+#define HIPCHK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t _r = (expr);                                                                    \
+        if (_r != hipSuccess)                                                                      \
+            return fail(e, PDDS_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_r));      \
+    } while (0)
+
+// grow-only device buffer for slot i
+pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
+    if (bytes == 0) bytes = 8;
+    DevBuf &b = e->bufs[slot];
+    if (b.n < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        if (hipMalloc(&b.p, bytes) != hipSuccess) return fail(e, PDDS_ERR_NOMEM, "device allocation failed");
+        b.n = bytes;
+    }
+    *out = b.p;
+    return PDDS_OK;
+}
+
+template <class T>
+pdds_status upload(pdds_engine *e, int slot, const std::vector<T> &v, T **out) {
+    void *p;
+    pdds_status s = dbuf(e, slot, v.size() * sizeof(T), &p);
+    if (s) return s;
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, e->stream));
+    *out = (T *)p;
+    return PDDS_OK;
+}
+
+bool valid_pde(int pde) { return pde >= PDDS_PDE_MONEGROS && pde <= PDDS_PDE_HELMHOLTZ; }
+
+pdds_status make_params(pdds_engine *e, const pdds_problem *p, PdeParams &pp) {
+    if (!p || !valid_pde(p->pde)) return fail(e, PDDS_ERR_INVALID, "invalid problem descriptor");
+    std::memset(&pp, 0, sizeof(pp));
+    pp.k2 = p->k2;
+    if (p->pde == PDDS_PDE_VARCOEF_LUT) {
+        if (!p->lut || p->lut->nf < 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
+        pp.lut = p->lut->view;
+    }
+    return PDDS_OK;
+}
+
+// host evaluation of the Dirichlet data (boundary knots: B = g(X0), Solvers/EMFKACSolver.hpp:720-727)
+double host_g(int pde, double x, double y, const PdeParams &pp) {
+    switch (pde) {
+        case PDE_MONEGROS: return pde_g<PDE_MONEGROS, double>(x, y, pp);
+        case PDE_SINSIN: return pde_g<PDE_SINSIN, double>(x, y, pp);
+        case PDE_HELMHOLTZ: return pde_g<PDE_HELMHOLTZ, double>(x, y, pp);
+        default: return pde_g<PDE_VARCOEF, double>(x, y, pp);
+    }
+}
+
+pdds_status check_run(pdds_engine *e, const pdds_run *r) {
+    if (!r) return fail(e, PDDS_ERR_INVALID, "null run parameters");
+    if (!(r->h > 0.0) || !std::isfinite(r->h)) return fail(e, PDDS_ERR_INVALID, "h must be positive");
+    if (r->precision != PDDS_FP64 && r->precision != PDDS_FP32) return fail(e, PDDS_ERR_INVALID, "bad precision");
+    if (r->wg_size != 0 && r->wg_size != 64 && r->wg_size != 128 && r->wg_size != 256)
+        return fail(e, PDDS_ERR_INVALID, "wg_size must be 64, 128 or 256");
+    if (r->sub_batch < 0 || r->item_paths < 0 || r->grid_wgs < 0 || r->max_steps < 0)
+        return fail(e, PDDS_ERR_INVALID, "negative launch parameter");
+    return PDDS_OK;
+}
+
+// One generic assembly over a list of knots sharing the stencil table.
+struct AsmKnot {
+    double x0, y0;
+    uint32_t id;
+    int stencil;
+    int64_t n;
+    int64_t out_off;
+};
+
+struct AsmOut {
+    std::vector<KnotStat> kstats;
+    double kernel_ms = 0, reduce_ms = 0;
+    int64_t items = 0;
+    int grid = 0, wg = 0, P = 0, S = 0;
+};
+
+int choose_P(int64_t n, int S) {
+    int64_t q = (n + 16 * (int64_t)S - 1) / (16 * (int64_t)S);
+    if (q < 1) q = 1;
+    if (q > 128) q = 128;
+    return (int)(q * S);
+}
+
+pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
+                         const std::vector<StencilDesc> &sten, const std::vector<double> &entx,
+                         const std::vector<double> &enty, const std::vector<int8_t> &ents,
+                         const std::vector<int32_t> &e1, const std::vector<int32_t> &e2,
+                         const std::vector<double> &ipsi, const std::vector<AsmKnot> &knots, int64_t n_vals,
+                         double *G_host, double *Gvar_host, AsmOut &out) {
+    PdeParams pp;
+    pdds_status st = make_params(e, prob, pp);
+    if (st) return st;
+    const int S = r->sub_batch > 0 ? r->sub_batch : 32;
+    const int wg = r->wg_size > 0 ? r->wg_size : 256;
+    const bool persistent = r->plain == 0;
+    const int64_t max_steps = r->max_steps > 0 ? r->max_steps : (int64_t)1 << 26;
+    if (!persistent && wg % S != 0) return fail(e, PDDS_ERR_INVALID, "plain mode needs wg_size to be a multiple of sub_batch");
+    if (r->item_paths > 0 && r->item_paths % S != 0)
+        return fail(e, PDDS_ERR_INVALID, "item_paths must be a multiple of sub_batch");
+    // items
+    std::vector<KnotDesc> kd(knots.size());
+    std::vector<ItemDesc> items;
+    int Pmax = 0;
+    int64_t part = 0;
+    int ne_cap = 1, nnz_cap = 1;
+    for (size_t q = 0; q < knots.size(); ++q) {
+        const AsmKnot &k = knots[q];
+        const StencilDesc &sd = sten[k.stencil];
+        const int P = !persistent ? wg : (r->item_paths > 0 ? r->item_paths : choose_P(k.n, S));
+        Pmax = std::max(Pmax, P);
+        ne_cap = std::max(ne_cap, sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3]);
+        nnz_cap = std::max(nnz_cap, sd.csr_nnz);
+        kd[q].x0 = k.x0; kd[q].y0 = k.y0; kd[q].knot_id = k.id; kd[q].stencil = k.stencil;
+        kd[q].n_paths = k.n; kd[q].out_off = k.out_off;
+        kd[q].item_begin = (int64_t)items.size();
+        for (int64_t p0 = 0; p0 < k.n; p0 += P) {
+            ItemDesc it;
+            it.knot = (int32_t)q;
+            it.n = (int32_t)std::min<int64_t>(P, k.n - p0);
+            it.p_begin = p0;
+            it.part_off = part;
+            part += 2 * (int64_t)sd.csr_nnz;
+            items.push_back(it);
+        }
+        kd[q].item_count = (int64_t)items.size() - kd[q].item_begin;
+    }
+    out.items = (int64_t)items.size();
+    out.S = S;
+    out.wg = wg;
+    out.P = Pmax;
+    out.kstats.assign(knots.size(), KnotStat{});
+    if (items.empty()) return PDDS_OK;
+    const size_t shmem = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap) * sizeof(double);
+    if (shmem > 160 * 1024) return fail(e, PDDS_ERR_UNSUPPORTED, "stencil too large for the LDS reduction buffers");
+    int grid;
+    if (persistent) {
+        grid = r->grid_wgs > 0 ? r->grid_wgs : 2 * e->n_cu;
+        grid = (int)std::min<int64_t>(grid, (int64_t)items.size());
+    } else {
+        grid = (int)items.size();
+    }
+    out.grid = grid;
+    // upload
+    EngineArgs a;
+    std::memset(&a, 0, sizeof(a));
+    KnotDesc *dk; StencilDesc *ds; double *dex, *dey, *dip; int8_t *des; int32_t *d1, *d2; ItemDesc *dit;
+    if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
+    if ((st = upload(e, B_STEN, sten, &ds))) return st;
+    if ((st = upload(e, B_ENTX, entx, &dex))) return st;
+    if ((st = upload(e, B_ENTY, enty, &dey))) return st;
+    if ((st = upload(e, B_ENTS, ents, &des))) return st;
+    if ((st = upload(e, B_E1, e1, &d1))) return st;
+    if ((st = upload(e, B_E2, e2, &d2))) return st;
+    if ((st = upload(e, B_IPSI, ipsi, &dip))) return st;
+    if ((st = upload(e, B_ITEMS, items, &dit))) return st;
+    void *p;
+    if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st;
+    a.stats = (ItemStat *)p;
+    if ((st = dbuf(e, B_PART, (size_t)part * sizeof(double), &p))) return st;
+    a.partials = (double *)p;
+    const int64_t rec_slots = persistent ? grid : (int64_t)items.size();
+    const size_t nrec = (size_t)rec_slots * (size_t)Pmax;
+    if ((st = dbuf(e, B_RECX, nrec * 8, &p))) return st; a.rec_x = (double *)p;
+    if ((st = dbuf(e, B_RECY, nrec * 8, &p))) return st; a.rec_y = (double *)p;
+    if ((st = dbuf(e, B_RECYY, nrec * 8, &p))) return st; a.rec_Y = (double *)p;
+    if ((st = dbuf(e, B_RECS, nrec * 8, &p))) return st; a.rec_score = (double *)p;
+    if ((st = dbuf(e, B_RECSIDE, nrec, &p))) return st; a.rec_side = (int8_t *)p;
+    if ((st = dbuf(e, B_COUNTER, 64, &p))) return st; a.work_counter = (unsigned long long *)p;
+    HIPCHK(hipMemsetAsync(a.work_counter, 0, 64, e->stream));
+    a.knots = dk; a.sten = ds; a.ent_x = dex; a.ent_y = dey; a.ent_side = des; a.csr_e1 = d1; a.csr_e2 = d2;
+    a.ipsi = dip; a.items = dit; a.n_items = (int64_t)items.size();
+    a.rec_stride = Pmax;
+    a.ne_cap = ne_cap; a.nnz_cap = nnz_cap;
+    for (int i = 0; i < 4; ++i) a.glob[i] = glob[i];
+    a.h = r->h; a.sqrth = std::sqrt(r->h); a.beta = r->beta;
+    a.seed = r->seed; a.stream = r->stream; a.max_steps = max_steps; a.S = S;
+    a.pp = pp;
+    double *dG, *dGv; KnotStat *dks;
+    if ((st = dbuf(e, B_G, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dG = (double *)p;
+    if ((st = dbuf(e, B_GVAR, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dGv = (double *)p;
+    if ((st = dbuf(e, B_KSTAT, knots.size() * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    hipError_t lr = r->precision == PDDS_FP64 ? launch_assemble_f64(prob->pde, persistent, grid, wg, shmem, e->stream, a)
+                                              : launch_assemble_f32(prob->pde, persistent, grid, wg, shmem, e->stream, a);
+    if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(lr));
+    HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    FinalizeArgs f;
+    f.knots = dk; f.sten = ds; f.items = dit; f.stats = a.stats; f.partials = a.partials;
+    f.n_knots = (int32_t)knots.size(); f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
+    lr = launch_finalize((int)knots.size(), 256, e->stream, f);
+    if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
+    HIPCHK(hipEventRecord(e->ev[2], e->stream));
+    if (n_vals > 0 && G_host) HIPCHK(hipMemcpyAsync(G_host, dG, (size_t)n_vals * 8, hipMemcpyDeviceToHost, e->stream));
+    if (n_vals > 0 && Gvar_host) HIPCHK(hipMemcpyAsync(Gvar_host, dGv, (size_t)n_vals * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(out.kstats.data(), dks, knots.size() * sizeof(KnotStat), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float ms1 = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms1, e->ev[0], e->ev[1]));
+    HIPCHK(hipEventElapsedTime(&ms2, e->ev[1], e->ev[2]));
+    out.kernel_ms = ms1;
+    out.reduce_ms = ms2;
+    return PDDS_OK;
+}
+
+// Stencil tables of a mesh (device layout), built per quirk mode.
+void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::vector<double> &entx,
+                 std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<int32_t> &e1,
+                 std::vector<int32_t> &e2, std::vector<double> &ipsi) {
+    for (const Stencil &s : m.stencils) {
+        StencilDesc d;
+        std::memset(&d, 0, sizeof(d));
+        for (int i = 0; i < 4; ++i) { d.rect[i] = s.rect[i]; d.cshape[i] = s.cshape[i]; d.msz[i] = (int)s.idx[i].size(); }
+        d.ent_off = (int32_t)entx.size();
+        d.csr_nnz = (int32_t)s.cols.size();
+        d.csr_off = (int32_t)e1.size();
+        d.gmask = s.gmask;
+        for (int i = 0; i < 4; ++i) {
+            d.ipsi_off[i] = (int64_t)ipsi.size();
+            ipsi.insert(ipsi.end(), s.ipsi[i].begin(), s.ipsi[i].end());
+            for (size_t j = 0; j < s.idx[i].size(); ++j) { entx.push_back(s.px[i][j]); enty.push_back(s.py[i][j]); ents.push_back((int8_t)i); }
+        }
+        const auto &a1 = quirk ? s.e1 : s.e1_sum;
+        for (size_t c = 0; c < s.cols.size(); ++c) {
+            e1.push_back(a1[c]);
+            e2.push_back(quirk ? -1 : s.e2_sum[c]);
+        }
+        sten.push_back(d);
+    }
+}
+
+void fill_counters(pdds_counters *ctr, const AsmOut &o, double total_ms) {
+    if (!ctr) return;
+    std::memset(ctr, 0, sizeof(*ctr));
+    for (const KnotStat &k : o.kstats) {
+        ctr->path_steps += (int64_t)k.steps;
+        ctr->capped += (int64_t)k.capped;
+        for (int q = 0; q < 6; ++q) ctr->exits[q] += (int64_t)k.cnt[q];
+    }
+    for (int q = 0; q < 6; ++q) ctr->paths += ctr->exits[q];
+    ctr->kernel_ms = o.kernel_ms;
+    ctr->reduce_ms = o.reduce_ms;
+    ctr->total_ms = total_ms;
+    ctr->work_items = o.items;
+    ctr->grid_wgs = o.grid;
+    ctr->wg_size = o.wg;
+    ctr->item_paths = o.P;
+    ctr->sub_batch = o.S;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pdds_status_string(pdds_status s) {
+    switch (s) {
+        case PDDS_OK: return "ok";
+        case PDDS_ERR_INVALID: return "invalid argument";
+        case PDDS_ERR_NOMEM: return "out of memory";
+        case PDDS_ERR_DEVICE: return "device error";
+        case PDDS_ERR_UNSUPPORTED: return "unsupported";
+        case PDDS_ERR_SINGULAR: return "singular interpolation matrix";
+        case PDDS_ERR_NOT_READY: return "not ready";
+        case PDDS_ERR_FULL: return "queue full or over budget";
+        default: return "unknown status";
+    }
+}
+
+void pdds_run_defaults(pdds_run *r) {
+    if (!r) return;
+    std::memset(r, 0, sizeof(*r));
+    r->h = 1e-3;
+    r->beta = -0.5826;
+    r->seed = 42;
+    r->precision = PDDS_FP64;
+    r->quirk_last_side = 1;
+    r->lut_lds = 1;
+}
+
+pdds_status pdds_engine_create(int device, pdds_engine **out) {
+    if (!out) return PDDS_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PDDS_ERR_DEVICE;
+    if (device < 0 || device >= n) return PDDS_ERR_INVALID;
+    pdds_engine *e = new pdds_engine();
+    e->device = device;
+    e->bufs.resize(B_NSLOTS);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return PDDS_ERR_DEVICE;
+    }
+    for (auto &ev : e->ev)
+        if (hipEventCreate(&ev) != hipSuccess) { delete e; return PDDS_ERR_DEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        e->n_cu = prop.multiProcessorCount;
+        e->name = prop.gcnArchName;
+    }
+    if (e->n_cu <= 0) e->n_cu = 256;
+    *out = e;
+    return PDDS_OK;
+}
+
+void pdds_engine_destroy(pdds_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &b : e->bufs)
+        if (b.p) (void)hipFree(b.p);
+    for (auto &ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char *pdds_last_error(const pdds_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+pdds_status pdds_engine_info(const pdds_engine *e, int32_t *n_cu, char *name, int32_t name_len) {
+    if (!e) return PDDS_ERR_INVALID;
+    if (n_cu) *n_cu = e->n_cu;
+    if (name && name_len > 0) {
+        std::strncpy(name, e->name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return PDDS_OK;
+}
+
+// ------------------------------------------------------------------ LUT ----
+pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, const double *samples, double x0,
+                            double y0, double x1, double y1, int32_t mode, pdds_lut **out) {
+    if (!e || !out || !samples) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    *out = nullptr;
+    if (nf < 1 || nf > 8 || nx < 4 || ny < 4 || !(x1 > x0) || !(y1 > y0) || (mode != 0 && mode != 1))
+        return fail(e, PDDS_ERR_INVALID, "invalid LUT shape");
+    std::vector<float> coef;
+    LutView v{};
+    int NX, NY;
+    if (mode == 0) {
+        NX = nx + 2; NY = ny + 2;
+        coef.resize((size_t)NX * NY * nf);
+        lut_prefilter(nx, ny, nf, samples, coef.data());
+        double dx = (x1 - x0) / (double)(nx - 1), dy = (y1 - y0) / (double)(ny - 1);
+        v.ox = x0 - dx; v.oy = y0 - dy; v.sx = 1.0 / dx; v.sy = 1.0 / dy;
+    } else {
+        NX = nx; NY = ny;
+        coef.resize((size_t)NX * NY * nf);
+        for (size_t q = 0; q < coef.size(); ++q) coef[q] = (float)samples[q];
+        v.sx = (double)nx / (x1 - x0); v.sy = (double)ny / (y1 - y0);
+        v.ox = x0 + 0.5 / v.sx; v.oy = y0 + 0.5 / v.sy;
+    }
+    v.nx = NX; v.ny = NY; v.nf = nf;
+    pdds_lut *l = new pdds_lut();
+    l->e = e;
+    if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, coef.size() * sizeof(float)) != hipSuccess) {
+        delete l;
+        return fail(e, PDDS_ERR_NOMEM, "LUT allocation failed");
+    }
+    if (hipMemcpy(l->d, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(l->d);
+        delete l;
+        return fail(e, PDDS_ERR_DEVICE, "LUT upload failed");
+    }
+    v.coef = l->d;
+    l->view = v;
+    l->nx = nx; l->ny = ny; l->nf = nf; l->mode = mode;
+    *out = l;
+    return PDDS_OK;
+}
+
+void pdds_lut_destroy(pdds_lut *l) {
+    if (!l) return;
+    if (l->d) (void)hipFree(l->d);
+    delete l;
+}
+
+pdds_status pdds_lut_eval(pdds_engine *e, const pdds_lut *l, int32_t field, int64_t n, const double *x,
+                          const double *y, double *out) {
+    if (!e || !l || !x || !y || !out || n < 0 || field < 0 || field >= l->nf) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (n == 0) return PDDS_OK;
+    void *px, *py, *po;
+    pdds_status st;
+    if ((st = dbuf(e, B_P0, (size_t)n * 8, &px)) || (st = dbuf(e, B_P1, (size_t)n * 8, &py)) ||
+        (st = dbuf(e, B_P2, (size_t)n * 8, &po)))
+        return st;
+    HIPCHK(hipMemcpyAsync(px, x, (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(py, y, (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_lut_eval(l->view, field, n, (const double *)px, (const double *)py, (double *)po, e->stream));
+    HIPCHK(hipMemcpyAsync(out, po, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return PDDS_OK;
+}
+
+// ----------------------------------------------------------------- mesh ----
+pdds_status pdds_mesh_create(double sw_x, double sw_y, double ne_x, double ne_y, int32_t n_sub, int32_t n_node,
+                             const double fac[4], pdds_mesh **out) {
+    if (!out || !fac) return PDDS_ERR_INVALID;
+    *out = nullptr;
+    if (n_sub < 2 || n_node < 4 || n_sub > 4096 || n_node > 100000 || !(ne_x > sw_x) || !(ne_y > sw_y))
+        return PDDS_ERR_INVALID;
+    for (int i = 0; i < 4; ++i)
+        if (!(fac[i] > 0.0)) return PDDS_ERR_INVALID;
+    pdds_mesh *m = new pdds_mesh(sw_x, sw_y, ne_x, ne_y, n_sub, n_node, fac);
+    *out = m;
+    return m->m.any_singular ? PDDS_ERR_SINGULAR : PDDS_OK;
+}
+
+void pdds_mesh_destroy(pdds_mesh *m) { delete m; }
+int32_t pdds_mesh_n_knots(const pdds_mesh *m) { return m ? m->m.n_knots() : 0; }
+int32_t pdds_mesh_n_stencils(const pdds_mesh *m) { return m ? (int32_t)m->m.stencils.size() : 0; }
+
+pdds_status pdds_mesh_knots(const pdds_mesh *m, double *x0, double *y0, double *px, double *py,
+                            int32_t *on_boundary, int32_t *stencil_id) {
+    if (!m) return PDDS_ERR_INVALID;
+    const Mesh &M = m->m;
+    for (int q = 0; q < M.n_knots(); ++q) {
+        if (x0) x0[q] = M.x0[q];
+        if (y0) y0[q] = M.y0[q];
+        if (px) px[q] = M.npx[q];
+        if (py) py[q] = M.npy[q];
+        if (on_boundary) on_boundary[q] = M.on_boundary[q];
+        if (stencil_id) stencil_id[q] = M.stencil_of[q];
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_mesh_stencil(const pdds_mesh *m, int32_t knot, double rect[4], int32_t msz[4], int32_t kind[4],
+                              int32_t cap, int32_t *idx, double *px, double *py, int32_t *n_entries) {
+    if (!m || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
+    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    int tot = s.total();
+    if (n_entries) *n_entries = tot;
+    for (int i = 0; i < 4; ++i) {
+        if (rect) rect[i] = s.rect[i];
+        if (msz) msz[i] = (int32_t)s.idx[i].size();
+        if (kind) kind[i] = s.kind[i];
+    }
+    if (idx || px || py) {
+        if (cap < tot) return PDDS_ERR_INVALID;
+        int o = 0;
+        for (int d = 0; d < 4; ++d)
+            for (size_t j = 0; j < s.idx[d].size(); ++j, ++o) {
+                if (idx) idx[o] = s.idx[d][j];
+                if (px) px[o] = s.px[d][j];
+                if (py) py[o] = s.py[d][j];
+            }
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_mesh_cond(const pdds_mesh *m, int32_t knot, double cond[4]) {
+    if (!m || !cond || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
+    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    for (int i = 0; i < 4; ++i) cond[i] = s.cond[i];
+    return PDDS_OK;
+}
+
+pdds_status pdds_mesh_ipsi(const pdds_mesh *m, int32_t knot, int32_t side, int64_t cap, double *out, int32_t *msz) {
+    if (!m || knot < 0 || knot >= m->m.n_knots() || side < 0 || side > 3) return PDDS_ERR_INVALID;
+    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    const int64_t mm = (int64_t)s.idx[side].size();
+    if (msz) *msz = (int32_t)mm;
+    if (out) {
+        if (cap < mm * mm) return PDDS_ERR_INVALID;
+        std::copy(s.ipsi[side].begin(), s.ipsi[side].end(), out);
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_mesh_csr(const pdds_mesh *m, int64_t *row_ptr, int32_t *cols, int64_t *nnz) {
+    if (!m) return PDDS_ERR_INVALID;
+    const Mesh &M = m->m;
+    if (nnz) *nnz = M.row_ptr.back();
+    if (row_ptr) std::copy(M.row_ptr.begin(), M.row_ptr.end(), row_ptr);
+    if (cols) std::copy(M.cols.begin(), M.cols.end(), cols);
+    return PDDS_OK;
+}
+
+// ------------------------------------------------------------- assembly ----
+pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                          const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var, double *B,
+                          double *BB, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    double t0 = now_ms();
+    if (!m || !B || !BB) return fail(e, PDDS_ERR_INVALID, "null mesh or output");
+    pdds_status st = check_run(e, r);
+    if (st) return st;
+    const Mesh &M = m->m;
+    if (M.any_singular) return fail(e, PDDS_ERR_SINGULAR, "mesh has a singular Psi");
+    PdeParams pp;
+    if ((st = make_params(e, p, pp))) return st;
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    std::vector<StencilDesc> sten;
+    std::vector<double> entx, enty, ipsi;
+    std::vector<int8_t> ents;
+    std::vector<int32_t> e1, e2;
+    mesh_tables(M, r->quirk_last_side, sten, entx, enty, ents, e1, e2, ipsi);
+    std::vector<AsmKnot> knots;
+    std::vector<int> kmap;
+    for (int q = 0; q < M.n_knots(); ++q) {
+        int64_t n = paths_per_knot ? paths_per_knot[q] : n_paths;
+        if (n < 0) return fail(e, PDDS_ERR_INVALID, "negative path count");
+        if (M.on_boundary[q]) {
+            B[q] = host_g(p->pde, M.x0[q], M.y0[q], pp);
+            BB[q] = B[q] * B[q];
+            continue;
+        }
+        if (n == 0) return fail(e, PDDS_ERR_INVALID, "interior knot with zero paths");
+        AsmKnot k{M.x0[q], M.y0[q], (uint32_t)q, M.stencil_of[q], n, M.row_ptr[q]};
+        knots.push_back(k);
+        kmap.push_back(q);
+    }
+    AsmOut o;
+    st = run_assembly(e, p, r, M.glob, sten, entx, enty, ents, e1, e2, ipsi, knots, M.row_ptr.back(), G, G_var, o);
+    if (st) return st;
+    for (size_t i = 0; i < knots.size(); ++i) { B[kmap[i]] = o.kstats[i].B; BB[kmap[i]] = o.kstats[i].BB; }
+    fill_counters(ctr, o, now_ms() - t0);
+    return PDDS_OK;
+}
+
+pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                            int32_t knot, int64_t n_paths, double *G, double *G_var, double *B, double *BB,
+                            pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    double t0 = now_ms();
+    if (!m || !B || !BB || knot < 0 || knot >= m->m.n_knots()) return fail(e, PDDS_ERR_INVALID, "bad knot");
+    pdds_status st = check_run(e, r);
+    if (st) return st;
+    const Mesh &M = m->m;
+    PdeParams pp;
+    if ((st = make_params(e, p, pp))) return st;
+    if (M.on_boundary[knot]) {
+        *B = host_g(p->pde, M.x0[knot], M.y0[knot], pp);
+        *BB = *B * *B;
+        if (ctr) std::memset(ctr, 0, sizeof(*ctr));
+        return PDDS_OK;
+    }
+    if (n_paths <= 0) return fail(e, PDDS_ERR_INVALID, "n_paths must be positive");
+    if (M.stencils[M.stencil_of[knot]].singular) return fail(e, PDDS_ERR_SINGULAR, "singular Psi");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    std::vector<StencilDesc> sten;
+    std::vector<double> entx, enty, ipsi;
+    std::vector<int8_t> ents;
+    std::vector<int32_t> e1, e2;
+    mesh_tables(M, r->quirk_last_side, sten, entx, enty, ents, e1, e2, ipsi);
+    std::vector<AsmKnot> knots = {AsmKnot{M.x0[knot], M.y0[knot], (uint32_t)knot, M.stencil_of[knot], n_paths, 0}};
+    AsmOut o;
+    st = run_assembly(e, p, r, M.glob, sten, entx, enty, ents, e1, e2, ipsi, knots,
+                      (int64_t)M.stencils[M.stencil_of[knot]].cols.size(), G, G_var, o);
+    if (st) return st;
+    *B = o.kstats[0].B;
+    *BB = o.kstats[0].BB;
+    fill_counters(ctr, o, now_ms() - t0);
+    return PDDS_OK;
+}
+
+pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
+                            int32_t n, const double *x, const double *y, const int64_t *n_paths, double *u,
+                            double *u_var, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    double t0 = now_ms();
+    if (!glob || !x || !y || !n_paths || !u || n < 0) return fail(e, PDDS_ERR_INVALID, "null argument");
+    pdds_status st = check_run(e, r);
+    if (st) return st;
+    if (!(glob[2] > glob[0]) || !(glob[3] > glob[1])) return fail(e, PDDS_ERR_INVALID, "bad rectangle");
+    PdeParams pp;
+    if ((st = make_params(e, p, pp))) return st;
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    StencilDesc d;
+    std::memset(&d, 0, sizeof(d));
+    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.cshape[i] = 1.0; }
+    d.gmask = 15;
+    std::vector<StencilDesc> sten = {d};
+    std::vector<double> entx, enty, ipsi;
+    std::vector<int8_t> ents;
+    std::vector<int32_t> e1, e2;
+    std::vector<AsmKnot> knots;
+    std::vector<int> kmap;
+    for (int i = 0; i < n; ++i) {
+        double ex, ey, nx, ny;
+        int kd;
+        double dist = rect_dist<double>(glob, x[i], y[i], ex, ey, nx, ny, kd);
+        if (std::fabs(dist) <= 1e-8 || dist > 0.0) {
+            // on (or outside) the boundary: the Dirichlet value, no Monte Carlo
+            u[i] = host_g(p->pde, ex, ey, pp);
+            if (u_var) u_var[i] = 0.0;
+            continue;
+        }
+        if (n_paths[i] <= 0) return fail(e, PDDS_ERR_INVALID, "n_paths must be positive");
+        knots.push_back(AsmKnot{x[i], y[i], (uint32_t)i, 0, n_paths[i], 0});
+        kmap.push_back(i);
+    }
+    AsmOut o;
+    st = run_assembly(e, p, r, glob, sten, entx, enty, ents, e1, e2, ipsi, knots, 0, nullptr, nullptr, o);
+    if (st) return st;
+    for (size_t i = 0; i < knots.size(); ++i) {
+        u[kmap[i]] = o.kstats[i].B;
+        if (u_var) u_var[kmap[i]] = o.kstats[i].BB - o.kstats[i].B * o.kstats[i].B;
+    }
+    fill_counters(ctr, o, now_ms() - t0);
+    return PDDS_OK;
+}
+
+// ---------------------------------------------------------------- paths ----
+pdds_status pdds_paths(pdds_engine *e, const pdds_problem *p, const pdds_run *r, double x0, double y0,
+                       const double rect[4], uint32_t knot, int64_t path0, int64_t n, int32_t phase2, double *xt,
+                       double *yt, double *Y, double *Z, double *xi, int64_t *steps, double *xt2, double *yt2,
+                       double *Y2, double *Z2, double *xi2, int64_t *steps2, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    double t0 = now_ms();
+    pdds_status st = check_run(e, r);
+    if (st) return st;
+    if (!rect || n < 0 || path0 < 0 || path0 + n > ((int64_t)1 << 32)) return fail(e, PDDS_ERR_INVALID, "bad path range");
+    if (!(rect[2] > rect[0]) || !(rect[3] > rect[1])) return fail(e, PDDS_ERR_INVALID, "bad rectangle");
+    PdeParams pp;
+    if ((st = make_params(e, p, pp))) return st;
+    if (n == 0) return PDDS_OK;
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    PathsArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x0 = x0; a.y0 = y0;
+    for (int i = 0; i < 4; ++i) a.rect[i] = rect[i];
+    a.knot = knot; a.seed = r->seed; a.stream = r->stream; a.phase2 = phase2 ? 1 : 0;
+    a.path0 = path0; a.n = n; a.h = r->h; a.sqrth = std::sqrt(r->h); a.beta = r->beta;
+    a.max_steps = r->max_steps > 0 ? r->max_steps : (int64_t)1 << 26;
+    a.pp = pp;
+    double *hd[10] = {xt, yt, Y, Z, xi, xt2, yt2, Y2, Z2, xi2};
+    double **dd[10] = {&a.xt, &a.yt, &a.Y, &a.Z, &a.xi, &a.xt2, &a.yt2, &a.Y2, &a.Z2, &a.xi2};
+    for (int i = 0; i < 10; ++i) {
+        if (!hd[i] || (i >= 5 && !phase2)) continue;
+        void *pp2;
+        if ((st = dbuf(e, B_P0 + i, (size_t)n * 8, &pp2))) return st;
+        *dd[i] = (double *)pp2;
+    }
+    // step counts always gathered (counters need them)
+    void *ps1, *ps2;
+    if ((st = dbuf(e, B_P10, (size_t)n * 8, &ps1))) return st;
+    a.steps = (int64_t *)ps1;
+    if (phase2) {
+        if ((st = dbuf(e, B_P11, (size_t)n * 8, &ps2))) return st;
+        a.steps2 = (int64_t *)ps2;
+    }
+    const int wg = r->wg_size > 0 ? r->wg_size : 256;
+    const int64_t grid = (n + wg - 1) / wg;
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    hipError_t lr = r->precision == PDDS_FP64 ? launch_paths_f64(p->pde, (int)grid, wg, e->stream, a)
+                                              : launch_paths_f32(p->pde, (int)grid, wg, e->stream, a);
+    if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("paths launch: ") + hipGetErrorString(lr));
+    HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    for (int i = 0; i < 10; ++i)
+        if (*dd[i]) HIPCHK(hipMemcpyAsync(hd[i], *dd[i], (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    std::vector<int64_t> hs((size_t)n);
+    HIPCHK(hipMemcpyAsync(hs.data(), a.steps, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    if (phase2 && steps2) HIPCHK(hipMemcpyAsync(steps2, a.steps2, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (steps) std::copy(hs.begin(), hs.end(), steps);
+    if (ctr) {
+        std::memset(ctr, 0, sizeof(*ctr));
+        for (int64_t s : hs) ctr->path_steps += s;
+        ctr->paths = n;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        ctr->kernel_ms = ms;
+        ctr->total_ms = now_ms() - t0;
+        ctr->wg_size = wg;
+        ctr->grid_wgs = (int32_t)grid;
+    }
+    return PDDS_OK;
+}
+
+}  // extern "C"
+
+extern "C" pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_ptr, const int32_t *cols,
+                                  const double *G, const double *B, double *u, double tol, int32_t maxit,
+                                  int32_t *iters, double *relres) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (n <= 0 || !row_ptr || !B || !u || (row_ptr[n] > 0 && (!cols || !G)) || !(tol >= 0.0))
+        return fail(e, PDDS_ERR_INVALID, "bad system");
+    const int64_t nnz = row_ptr[n];
+    for (int i = 0; i < n; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(e, PDDS_ERR_INVALID, "row_ptr not monotone");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (cols[k] < 0 || cols[k] >= n) return fail(e, PDDS_ERR_INVALID, "column out of range");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    void *drp, *dci, *dv, *db, *dx, *dw, *dit;
+    pdds_status st;
+    if ((st = dbuf(e, B_P0, (size_t)(n + 1) * 8, &drp)) || (st = dbuf(e, B_P1, (size_t)std::max<int64_t>(nnz, 1) * 4, &dci)) ||
+        (st = dbuf(e, B_P2, (size_t)std::max<int64_t>(nnz, 1) * 8, &dv)) || (st = dbuf(e, B_P3, (size_t)n * 8, &db)) ||
+        (st = dbuf(e, B_P4, (size_t)n * 8, &dx)) || (st = dbuf(e, B_P5, (size_t)n * 6 * 8, &dw)) ||
+        (st = dbuf(e, B_P6, 64, &dit)))
+        return st;
+    HIPCHK(hipMemcpyAsync(drp, row_ptr, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, e->stream));
+    if (nnz > 0) {
+        HIPCHK(hipMemcpyAsync(dci, cols, (size_t)nnz * 4, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(dv, G, (size_t)nnz * 8, hipMemcpyHostToDevice, e->stream));
+    }
+    HIPCHK(hipMemcpyAsync(db, B, (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
+    const int mi = maxit > 0 ? maxit : 2 * n;
+    HIPCHK(bicgstab_device(n, (const int64_t *)drp, (const int32_t *)dci, (const double *)dv, (const double *)db,
+                           (double *)dx, (double *)dw, tol, mi, (int *)dit, (double *)((char *)dit + 8), e->stream));
+    struct { int it; int pad; double rr; } hr;
+    HIPCHK(hipMemcpyAsync(u, dx, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(&hr, dit, 16, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (iters) *iters = hr.it;
+    if (relres) *relres = hr.rr;
+    return PDDS_OK;
+}

@@ -1,0 +1,116 @@
+// engine_types.h -- device-resident descriptors shared by host and kernels.
+#pragma once
+#include <stdint.h>
+#include "pdds_pde.h"
+
+namespace pdds {
+
+constexpr int kNBG = 8;  // sub-batches reduced per pass over Psi^-1
+
+// One Monte Carlo knot of an assembly launch.
+struct KnotDesc {
+    double x0, y0;          // start point
+    uint32_t knot_id;       // Philox counter word 2 (global knot index)
+    int32_t stencil;        // index into the stencil table
+    int64_t n_paths;
+    int64_t out_off;        // first value of the knot's G row in the CSR value array
+    int64_t item_begin;     // first work item of the knot
+    int64_t item_count;
+};
+
+// One distinct stencil: stopping rectangle, four side lists (N,S,E,W), Psi^-1 per side,
+// and for each CSR column of the row up to two contributing side entries.
+struct StencilDesc {
+    double rect[4];
+    double cshape[4];       // Gaussian shape c per side
+    int64_t ipsi_off[4];    // row-major m x m blocks in the ipsi array
+    int32_t msz[4];
+    int32_t ent_off;        // first side entry in ent_x / ent_y / ent_side
+    int32_t csr_nnz;        // columns in the CSR row
+    int32_t csr_off;        // first column map entry in csr_e1 / csr_e2
+    int32_t gmask;          // bit s set: side s lies on the global boundary
+};
+
+struct ItemDesc {
+    int32_t knot;           // index into the knot table
+    int32_t n;              // paths in the item
+    int64_t p_begin;        // first path index (Philox counter word 1)
+    int64_t part_off;       // 2*nnz partials (sum, sum of squares / n_beta)
+};
+
+struct ItemStat {
+    double Bs, BBs;
+    unsigned long long steps;
+    uint32_t cnt[6];
+    uint32_t capped;
+    uint32_t pad;
+};
+
+struct KnotStat {
+    double B, BB;
+    unsigned long long steps;
+    unsigned long long capped;
+    unsigned long long cnt[6];
+};
+
+struct EngineArgs {
+    const KnotDesc *knots;
+    const StencilDesc *sten;
+    const double *ent_x, *ent_y;
+    const int8_t *ent_side;
+    const int32_t *csr_e1, *csr_e2;
+    const double *ipsi;
+    const ItemDesc *items;
+    int64_t n_items;
+    ItemStat *stats;
+    double *partials;
+    double *rec_x, *rec_y, *rec_Y, *rec_score;
+    int8_t *rec_side;
+    int64_t rec_stride;
+    unsigned long long *work_counter;
+    int32_t ne_cap, nnz_cap;
+    double glob[4];
+    double h, sqrth, beta;
+    uint32_t seed, stream;
+    int64_t max_steps;
+    int32_t S;
+    PdeParams pp;
+};
+
+struct FinalizeArgs {
+    const KnotDesc *knots;
+    const StencilDesc *sten;
+    const ItemDesc *items;
+    const ItemStat *stats;
+    const double *partials;
+    int32_t n_knots;
+    int32_t S;
+    double *G, *Gvar;
+    KnotStat *kstats;
+};
+
+struct PathsArgs {
+    double x0, y0;
+    double rect[4];
+    uint32_t knot, seed, stream;
+    int32_t phase2;
+    int64_t path0, n;
+    double h, sqrth, beta;
+    int64_t max_steps;
+    PdeParams pp;
+    double *xt, *yt, *Y, *Z, *xi;
+    int64_t *steps;
+    double *xt2, *yt2, *Y2, *Z2, *xi2;
+    int64_t *steps2;
+};
+
+// Launchers (defined in engine_f64.hip / engine_f32.hip / engine_common.hip)
+hipError_t launch_assemble_f64(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a);
+hipError_t launch_assemble_f32(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a);
+hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);
+hipError_t launch_paths_f32(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);
+hipError_t launch_finalize(int grid, int wg, hipStream_t s, const FinalizeArgs &f);
+hipError_t launch_lut_eval(const LutView &L, int field, int64_t n, const double *x, const double *y, double *out,
+                           hipStream_t s);
+
+}  // namespace pdds

@@ -1,0 +1,84 @@
+// mesh.h -- host-side interface/knot/stencil bookkeeping of PDDSparse.
+//
+// C++ restatement of PDDSparseGM's geometry for the main_EXTENDED
+// configuration (INTERSECTIONS_YES, STEN_ELONG = 0.5, square grids):
+//   Fullfill_interfaces     PDDAlgorithms/PDDSparseGM.cpp:346-689
+//   Solve's knot dispatch   PDDAlgorithms/PDDSparseGM.cpp:705-756
+//   Node_Position           PDDAlgorithms/PDDSparseGM.cpp:3156-3167
+//   Interface::N/S/E/W      Meshes/interface.cpp:16-93
+//   Labels_Stencil          PDDAlgorithms/PDDSparseGM.cpp:1330-1433
+//   Compute_Stencil         PDDAlgorithms/PDDSparseGM.cpp:1845-2041
+//   Stencil::Init           Meshes/stencil.cpp:11-85
+//   Stencil::Compute_ipsi   Meshes/stencil.cpp:115-243 (full-pivot LU inverse, Frobenius cond)
+// Knots sharing an identical stencil (all interior knots of a segment) share one
+// Psi^-1, computed once.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace pdds {
+
+enum Dir { North = 0, South = 1, East = 2, West = 3 };
+
+struct Interface {
+    std::array<int, 2> label;
+    std::vector<int> index;
+    std::vector<double> px, py;
+    bool In(int k) const;
+    std::vector<std::array<int, 2>> N() const;
+    std::vector<std::array<int, 2>> S() const;
+    std::vector<std::array<int, 2>> E() const;
+    std::vector<std::array<int, 2>> W() const;
+};
+
+struct Stencil {
+    double rect[4];
+    int kind[4];
+    std::vector<int> idx[4];
+    std::vector<double> px[4], py[4];
+    // derived
+    double cshape[4] = {1, 1, 1, 1};
+    double cond[4] = {0, 0, 0, 0};
+    std::vector<double> ipsi[4];          // row-major m x m
+    std::vector<int> cols;                 // sorted unique CSR columns
+    std::vector<int> e1, e2;               // per column: contributing side entries (concatenated N,S,E,W), -1 none
+    std::vector<int> e1_sum, e2_sum;       // same, summing shared corners
+    int gmask = 0;
+    bool singular = false;
+    int total() const { return (int)(idx[0].size() + idx[1].size() + idx[2].size() + idx[3].size()); }
+    bool same_as(const Stencil &o) const;
+};
+
+class Mesh {
+  public:
+    Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_node, const double fac[4]);
+    int n_knots() const { return nknots_; }
+    const std::vector<Interface> &interfaces() const { return ifs_; }
+    double glob[4];
+    std::vector<double> x0, y0, npx, npy;   // start / reported positions
+    std::vector<int> on_boundary;
+    std::vector<int> stencil_of;            // knot -> distinct stencil id
+    std::vector<Stencil> stencils;
+    std::vector<int64_t> row_ptr;           // CSR of G (no identity); boundary knots: empty rows
+    std::vector<int> cols;
+    bool any_singular = false;
+    double fac[4];
+
+  private:
+    int find(int a, int b) const;
+    std::map<Dir, std::vector<std::array<int, 2>>> labels_stencil(int k) const;
+    Stencil compute_stencil(int k) const;
+    int n_, k_;
+    int nknots_ = 0;
+    std::vector<Interface> ifs_;
+    std::map<std::array<int, 2>, int> imap_;
+};
+
+// Full-pivot LU inverse of the Gaussian RBF matrix; returns false if singular.
+bool psi_inverse(const std::vector<double> &px, const std::vector<double> &py, double c, std::vector<double> &ipsi,
+                 double &cond);
+
+}  // namespace pdds

@@ -1,0 +1,362 @@
+// pdds_math.h -- device/host arithmetic for the path engine.
+//
+// FP64: reproducible elementary functions built only from IEEE +,-,*,/,sqrt
+// and explicit fma in a fixed order (TUs that include this for FP64 are
+// compiled with -ffp-contract=off), so every FP64 path is a fixed function of
+// its Philox stream.  Recipes (textbook): exp by two-part ln2 reduction and
+// the Taylor series of e^r to degree 14; log by 2 atanh((m-1)/(m+1)) to
+// s^27; sinpi/cospi by exact reduction to |r| <= 1/4 and Taylor series to
+// degree 21/22; sin/cos through a two-part 1/pi; tanh through expm1.
+// FP32: the CDNA hardware transcendentals (v_exp_f32, v_log_f32, v_sin_f32,
+// v_cos_f32 take base-2 / revolution arguments) and v_rcp_f32.
+//
+// Philox4x32-10 after Salmon et al., SC'11 (public multipliers and Weyl
+// constants), 64-bit products through v_mad_u64_u32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PDDS_HD __host__ __device__ __forceinline__
+
+namespace pdds {
+
+// ------------------------------------------------------------------ FP64 --
+PDDS_HD uint64_t f64_bits(double x) { return (uint64_t)__builtin_bit_cast(uint64_t, x); }
+PDDS_HD double f64_from(uint64_t u) { return __builtin_bit_cast(double, u); }
+PDDS_HD double pow2i(int k) { return f64_from((uint64_t)(k + 1023) << 52); }
+
+constexpr double kLn2Hi = 0.6931467056274414;
+constexpr double kLn2Lo = 4.7493250390316726e-07;
+constexpr double kLog2e = 1.4426950408889634;
+constexpr double kPi = 3.141592653589793;
+constexpr double kPiLo = 1.2246467991473532e-16;
+constexpr double kInvPiHi = 0.3183098861837907;
+constexpr double kInvPiLo = -1.9678676675182486e-17;
+
+PDDS_HD double exp_poly(double r) {
+    double p = 1.1470745597729725e-11;
+    p = __builtin_fma(p, r, 1.6059043836821613e-10);
+    p = __builtin_fma(p, r, 2.08767569878681e-09);
+    p = __builtin_fma(p, r, 2.505210838544172e-08);
+    p = __builtin_fma(p, r, 2.755731922398589e-07);
+    p = __builtin_fma(p, r, 2.7557319223985893e-06);
+    p = __builtin_fma(p, r, 2.48015873015873e-05);
+    p = __builtin_fma(p, r, 0.0001984126984126984);
+    p = __builtin_fma(p, r, 0.001388888888888889);
+    p = __builtin_fma(p, r, 0.008333333333333333);
+    p = __builtin_fma(p, r, 0.041666666666666664);
+    p = __builtin_fma(p, r, 0.16666666666666666);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return p;
+}
+
+PDDS_HD double dexp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return __builtin_huge_val();
+    if (x < -745.2) return 0.0;
+    double k = __builtin_floor(x * kLog2e + 0.5);
+    double r = __builtin_fma(-k, kLn2Hi, x);
+    r = __builtin_fma(-k, kLn2Lo, r);
+    double p = exp_poly(r);
+    int ki = (int)k;
+    if (ki > 1023) return (p * 2.0) * pow2i(ki - 1);
+    if (ki >= -1022) return p * pow2i(ki);
+    return (p * pow2i(ki + 600)) * pow2i(-600);
+}
+
+PDDS_HD double dexpm1(double x) {
+    if (__builtin_fabs(x) <= 0.5) {
+        double p = 2.8114572543455206e-15;
+        p = __builtin_fma(p, x, 4.779477332387385e-14);
+        p = __builtin_fma(p, x, 7.647163731819816e-13);
+        p = __builtin_fma(p, x, 1.1470745597729725e-11);
+        p = __builtin_fma(p, x, 1.6059043836821613e-10);
+        p = __builtin_fma(p, x, 2.08767569878681e-09);
+        p = __builtin_fma(p, x, 2.505210838544172e-08);
+        p = __builtin_fma(p, x, 2.755731922398589e-07);
+        p = __builtin_fma(p, x, 2.7557319223985893e-06);
+        p = __builtin_fma(p, x, 2.48015873015873e-05);
+        p = __builtin_fma(p, x, 0.0001984126984126984);
+        p = __builtin_fma(p, x, 0.001388888888888889);
+        p = __builtin_fma(p, x, 0.008333333333333333);
+        p = __builtin_fma(p, x, 0.041666666666666664);
+        p = __builtin_fma(p, x, 0.16666666666666666);
+        p = __builtin_fma(p, x, 0.5);
+        p = __builtin_fma(p, x, 1.0);
+        return p * x;
+    }
+    return dexp(x) - 1.0;
+}
+
+PDDS_HD double dlog(double x) {
+    if (!(x > 0.0)) return (x == 0.0) ? -__builtin_huge_val() : __builtin_nan("");
+    if (x == __builtin_huge_val()) return x;
+    uint64_t b = f64_bits(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {
+        x *= 18014398509481984.0;
+        b = f64_bits(x);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = f64_from((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 0.07407407407407407;
+    p = __builtin_fma(p, s2, 0.08);
+    p = __builtin_fma(p, s2, 0.08695652173913043);
+    p = __builtin_fma(p, s2, 0.09523809523809523);
+    p = __builtin_fma(p, s2, 0.10526315789473684);
+    p = __builtin_fma(p, s2, 0.11764705882352941);
+    p = __builtin_fma(p, s2, 0.13333333333333333);
+    p = __builtin_fma(p, s2, 0.15384615384615385);
+    p = __builtin_fma(p, s2, 0.18181818181818182);
+    p = __builtin_fma(p, s2, 0.2222222222222222);
+    p = __builtin_fma(p, s2, 0.2857142857142857);
+    p = __builtin_fma(p, s2, 0.4);
+    p = __builtin_fma(p, s2, 0.6666666666666666);
+    p = __builtin_fma(p, s2, 2.0);
+    double lm = s * p;
+    double de = (double)e;
+    return __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, lm));
+}
+
+PDDS_HD void sincospi_kernel(double r, double &s, double &c) {
+    double y = __builtin_fma(r, kPi, r * kPiLo);
+    double y2 = y * y;
+    double ps = 1.9572941063391263e-20;
+    ps = __builtin_fma(ps, y2, -8.22063524662433e-18);
+    ps = __builtin_fma(ps, y2, 2.8114572543455206e-15);
+    ps = __builtin_fma(ps, y2, -7.647163731819816e-13);
+    ps = __builtin_fma(ps, y2, 1.6059043836821613e-10);
+    ps = __builtin_fma(ps, y2, -2.505210838544172e-08);
+    ps = __builtin_fma(ps, y2, 2.7557319223985893e-06);
+    ps = __builtin_fma(ps, y2, -0.0001984126984126984);
+    ps = __builtin_fma(ps, y2, 0.008333333333333333);
+    ps = __builtin_fma(ps, y2, -0.16666666666666666);
+    s = __builtin_fma(ps * y2, y, y);
+    double pc = -8.896791392450574e-22;
+    pc = __builtin_fma(pc, y2, 4.110317623312165e-19);
+    pc = __builtin_fma(pc, y2, -1.5619206968586225e-16);
+    pc = __builtin_fma(pc, y2, 4.779477332387385e-14);
+    pc = __builtin_fma(pc, y2, -1.1470745597729725e-11);
+    pc = __builtin_fma(pc, y2, 2.08767569878681e-09);
+    pc = __builtin_fma(pc, y2, -2.755731922398589e-07);
+    pc = __builtin_fma(pc, y2, 2.48015873015873e-05);
+    pc = __builtin_fma(pc, y2, -0.001388888888888889);
+    pc = __builtin_fma(pc, y2, 0.041666666666666664);
+    pc = __builtin_fma(pc, y2, -0.5);
+    c = __builtin_fma(pc, y2, 1.0);
+}
+
+// sin(pi x), cos(pi x); |x| < 2^52 assumed (path coordinates and uniforms)
+PDDS_HD void dsincospi(double x, double &sp, double &cp) {
+    double k = __builtin_floor(2.0 * x + 0.5);
+    double r = x - 0.5 * k;
+    double s, c;
+    sincospi_kernel(r, s, c);
+    int q = ((int)(int64_t)k) & 3;
+    double a = (q & 1) ? c : s;
+    double b = (q & 1) ? -s : c;
+    sp = (q & 2) ? -a : a;
+    cp = (q & 2) ? -b : b;
+}
+PDDS_HD double x_over_pi(double x) { return __builtin_fma(x, kInvPiHi, x * kInvPiLo); }
+PDDS_HD void dsincos(double x, double &s, double &c) { dsincospi(x_over_pi(x), s, c); }
+PDDS_HD double dsin(double x) { double s, c; dsincos(x, s, c); return s; }
+PDDS_HD double dcos(double x) { double s, c; dsincos(x, s, c); return c; }
+PDDS_HD double dtanh(double x) {
+    if (x > 20.0) return 1.0;
+    if (x < -20.0) return -1.0;
+    double e = dexpm1(2.0 * x);
+    return e / (e + 2.0);
+}
+
+// --------------------------------------------------------------- traits --
+template <class R> struct RM;
+template <> struct RM<double> {
+    static PDDS_HD double exp(double x) { return dexp(x); }
+    static PDDS_HD double log(double x) { return dlog(x); }
+    static PDDS_HD double sqrt(double x) { return __builtin_sqrt(x); }
+    static PDDS_HD void sincospi(double x, double &s, double &c) { dsincospi(x, s, c); }
+    static PDDS_HD void sincos(double x, double &s, double &c) { dsincos(x, s, c); }
+    static PDDS_HD double sin(double x) { return dsin(x); }
+    static PDDS_HD double cos(double x) { return dcos(x); }
+    static PDDS_HD double tanh(double x) { return dtanh(x); }
+    static PDDS_HD double div(double a, double b) { return a / b; }
+    static PDDS_HD double floor(double x) { return __builtin_floor(x); }
+    static PDDS_HD double fmin(double a, double b) { return __builtin_fmin(a, b); }
+    static PDDS_HD double fmax(double a, double b) { return __builtin_fmax(a, b); }
+    static PDDS_HD double fabs(double a) { return __builtin_fabs(a); }
+};
+template <> struct RM<float> {
+    // v_exp_f32 computes 2^x, v_log_f32 log2, v_sin/v_cos_f32 take revolutions
+    static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+    static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+    static __device__ __forceinline__ void sincospi(float x, float &s, float &c) {
+        float t = x * 0.5f;
+        t = t - __builtin_floorf(t);
+        s = __builtin_amdgcn_sinf(t);
+        c = __builtin_amdgcn_cosf(t);
+    }
+    static __device__ __forceinline__ void sincos(float x, float &s, float &c) {
+        float t = x * 0.15915494309189535f;
+        t = t - __builtin_floorf(t);
+        s = __builtin_amdgcn_sinf(t);
+        c = __builtin_amdgcn_cosf(t);
+    }
+    static __device__ __forceinline__ float sin(float x) { float s, c; sincos(x, s, c); return s; }
+    static __device__ __forceinline__ float cos(float x) { float s, c; sincos(x, s, c); return c; }
+    static __device__ __forceinline__ float tanh(float x) {
+        float ax = __builtin_fabsf(x);
+        float e = exp(-2.0f * ax);
+        float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
+        return x < 0.0f ? -t : t;
+    }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
+    static __device__ __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
+    static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+    static __device__ __forceinline__ float fabs(float a) { return __builtin_fabsf(a); }
+};
+
+// ---------------------------------------------------------------- Philox --
+struct Philox4 { uint32_t v[4]; };
+PDDS_HD Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    Philox4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+PDDS_HD double u01_53(uint32_t a, uint32_t b) {
+    uint64_t m = ((uint64_t)a << 21) | (uint64_t)(b >> 11);
+    return ((double)m + 0.5) * 1.1102230246251565e-16;
+}
+
+// FP64 normal pair for one step: one Philox draw, Box-Muller on 53-bit uniforms.
+PDDS_HD void normals_f64(uint32_t seed, uint32_t stream, uint32_t knot, uint32_t path, int64_t step,
+                         double &n1, double &n2) {
+    Philox4 w = philox4x32_10((uint32_t)step, path, knot, (uint32_t)(step >> 32), seed, stream);
+    double u1 = u01_53(w.v[0], w.v[1]);
+    double u2 = u01_53(w.v[2], w.v[3]);
+    double r = __builtin_sqrt(-2.0 * dlog(u1));
+    double s, c;
+    dsincospi(2.0 * u2, s, c);
+    n1 = r * c;
+    n2 = r * s;
+}
+
+// ------------------------------------------------------------ Rectangle --
+// Signed distance to p = [x0,y0,x1,y1] with nearest-side projection and outward
+// normal; centre/half-side form with the quadrant split and tie rule of the
+// reference (BVPs/Domains/rectangle.hpp:7-229).  `kind` reports the projection
+// as an integer: 0 N, 1 S, 2 E, 3 W side; 4+ corner (4 NE, 5 NW, 6 SE, 7 SW).
+template <class R>
+PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kind) {
+    const R cx = (R)0.5 * (p[0] + p[2]), cy = (R)0.5 * (p[1] + p[3]);
+    const R hx = p[2] - cx, hy = p[3] - cy;
+    const R dx = RM<R>::fabs(x - cx), dy = RM<R>::fabs(y - cy);
+    const R diag = (R)0.70710678118f;
+    const bool right = x > cx, top = y > cy;
+    const R sxs = right ? (R)1 : (R)-1;
+    const R sys = top ? (R)1 : (R)-1;
+    ex = x; ey = y;
+    if (dx <= hx && dy <= hy) {
+        if (dx - hx > dy - hy) {
+            ex = sxs * hx + cx; nx = sxs; ny = 0;
+            kind = right ? 2 : 3;
+            return dx - hx;
+        }
+        ey = sys * hy + cy; nx = 0; ny = sys;
+        kind = top ? 0 : 1;
+        return dy - hy;
+    }
+    if (dx > hx) {
+        if (dy > hy) {
+            R r = RM<R>::sqrt((dx - hx) * (dx - hx) + (dy - hy) * (dy - hy));
+            ex = sxs * hx + cx; ey = sys * hy + cy;
+            nx = sxs * diag; ny = sys * diag;
+            kind = top ? (right ? 4 : 5) : (right ? 6 : 7);
+            return r;
+        }
+        ex = sxs * hx + cx; nx = sxs; ny = 0;
+        kind = right ? 2 : 3;
+        return dx - hx;
+    }
+    ey = sys * hy + cy; nx = 0; ny = sys;
+    kind = top ? 0 : 1;
+    return dy - hy;
+}
+
+// --------------------------------------------------------------- LUT -----
+// Cubic B-spline coefficient table [ny][nx][nf] (FP32), t = (x - ox) * sx.
+struct LutView {
+    const float *coef;
+    int nx, ny, nf;
+    double ox, oy, sx, sy;
+};
+
+template <class R> PDDS_HD void bspline_w(R f, R w[4]) {
+    R f2 = f * f, f3 = f2 * f, om = (R)1 - f;
+    const R s6 = (R)(1.0 / 6.0);
+    w[0] = (om * om * om) * s6;
+    w[1] = ((R)3 * f3 - (R)6 * f2 + (R)4) * s6;
+    w[2] = ((R)-3 * f3 + (R)3 * f2 + (R)3 * f + (R)1) * s6;
+    w[3] = f3 * s6;
+}
+
+// Evaluate fields [0, NF) at (x,y): sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
+template <class R, int NF>
+PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
+    R t = (x - (R)L.ox) * (R)L.sx;
+    R v = (y - (R)L.oy) * (R)L.sy;
+    t = RM<R>::fmin(RM<R>::fmax(t, (R)-2), (R)L.nx + (R)1);
+    v = RM<R>::fmin(RM<R>::fmax(v, (R)-2), (R)L.ny + (R)1);
+    R fi = RM<R>::floor(t), fj = RM<R>::floor(v);
+    R wx[4], wy[4];
+    bspline_w<R>(t - fi, wx);
+    bspline_w<R>(v - fj, wy);
+    int i = (int)fi, j = (int)fj;
+    R acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = (R)0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int jj = j - 1 + b;
+        jj = jj < 0 ? 0 : (jj > L.ny - 1 ? L.ny - 1 : jj);
+        R row[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) row[f] = (R)0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            int ii = i - 1 + a;
+            ii = ii < 0 ? 0 : (ii > L.nx - 1 ? L.nx - 1 : ii);
+            const float *cp = L.coef + ((int64_t)jj * L.nx + ii) * L.nf;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) row[f] = row[f] + wx[a] * (R)cp[f];
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) acc[f] = acc[f] + wy[b] * row[f];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) out[f] = acc[f];
+}
+
+}  // namespace pdds

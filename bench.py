@@ -1,0 +1,188 @@
+"""Benchmark: PDDSparse Monte Carlo assembly on one MI355X.
+
+Workload (default): plain PDDSparse at GPU scale -- divergence-form variable-coefficient
+elliptic problem on [-1,1]^2 (sigma = diag(1+0.3 sin x, 1+0.3 cos y), b = (0.2y, -0.2x),
+c = 1 + x^2, u = e^x cos y), 8x8 subdomains x 32 knots per interface (3437 knots, 3409 of
+them Monte Carlo knots), 100,000 paths per knot, h = 1e-4, FP64 paths, sigma/b/c served
+from 512x512 bicubic B-spline LUTs, Philox seed 7.  One step = one full assembly of G (CSR)
+and B over every knot.  Synthetic input: the manufactured problem itself.
+
+Prints one JSON line (rank 0).  `value` is whole-job phase-1 path-steps per second over the
+K timed assemblies (inputs resident on the device).  Extra fields report the plain FP64 GPU
+form, the FP32-path engine and the RMS interface error of each.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 operation count per path-step of the LUT configuration (+,-,x = 1,
+# fma = 2): LUT 264 (coordinates 6, weights 18, 5 fields x 40 + row sums), grad u 2, xi 7,
+# Z 3, Y 3, X 8, Gobet 5, Box-Muller 10.  Transcendentals and Philox integer work are
+# counted separately and are not in this figure.
+FLOPS_PER_STEP = {"cfg2_varcoef_lut": 302.0, "cfg1_poisson": 41.0, "fixture": 41.0}
+PEAK_FP64_TFLOPS = 78.6
+
+
+def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
+    """The FP64 C/OpenMP oracle (a restatement of the reference CPU path) on a bounded sample:
+    a seeded random subset of knots, paths per knot scaled to about budget_s seconds."""
+    from oracle import oracle as O
+    from pdds_amd.workloads import varcoef_fields
+    threads = os.cpu_count() or 1
+    if prob_name == "varcoef_lut":
+        n = 512
+        xs = np.linspace(-1, 1, n)
+        X, Y = np.meshgrid(xs, xs)
+        coef = O.lut_prefilter(varcoef_fields(X, Y))
+        d = 2.0 / (n - 1)
+        lut = {"coef": coef, "NX": n + 2, "NY": n + 2, "nf": 5, "ox": -1 - d, "oy": -1 - d, "sx": 1 / d, "sy": 1 / d}
+        pb = O.problem(O.PDE_VARCOEF_LUT, lut=lut)
+    else:
+        pb = O.problem({"monegros": O.PDE_MONEGROS, "sinsin": O.PDE_SINSIN}[prob_name])
+    rng = np.random.default_rng(1234)
+    interior = np.flatnonzero(~mesh.on_boundary)
+    knots = rng.choice(interior, size=min(8, len(interior)), replace=False)
+    run = O.run_params(h, seed, threads=threads)
+    # calibrate on a small batch, then size the sample
+    k0 = int(knots[0])
+    st0 = mesh.stencil(k0)
+    t = time.perf_counter()
+    cal = O.simulate(pb, run, mesh.x0[k0], mesh.y0[k0], st0["rect"], k0, 0, 256)
+    rate = cal["steps"].sum() / max(time.perf_counter() - t, 1e-6)
+    steps_per_path = max(cal["steps"].mean(), 1.0)
+    per_knot = int(max(256, min(n_paths_ref, budget_s * rate / (len(knots) * steps_per_path))))
+    steps = 0
+    t = time.perf_counter()
+    for k in knots:
+        k = int(k)
+        st = mesh.stencil(k)
+        paths = O.simulate(pb, run, mesh.x0[k], mesh.y0[k], st["rect"], k, 0, per_knot)
+        steps += int(paths["steps"].sum())
+    t_sim = time.perf_counter() - t
+    return {"value": steps / t_sim, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d knots x %d paths (seeded subset of %d interior knots), phase-1 stepping, "
+                      "FP64 C/OpenMP oracle, %.1f s" % (len(knots), per_knot, len(interior), t_sim),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def rms_error(eng, mesh, prob, out):
+    u, it, rr = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, out["G"], out["B"], tol=1e-12)
+    ex = prob.u(mesh.px, mesh.py)
+    return float(np.sqrt(np.mean((u - ex) ** 2))), it
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2_varcoef_lut")
+    ap.add_argument("--paths", type=int, default=0, help="override paths per knot (0: the config's)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_cfg2.json"))
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the path engine has no multi-GPU form in this build "
+              "(--gpus %d, WORLD_SIZE %d requested)" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    from pdds_amd.engine import Engine, default_run
+    from pdds_amd import workloads as W
+
+    eng = Engine(0)
+    info = eng.info()
+    mesh, prob, n_paths, h, seed = W.build(args.config, eng)
+    if args.paths:
+        n_paths = args.paths
+    run = default_run(h=h, seed=seed)
+
+    for _ in range(args.warmup):
+        eng.assemble(mesh, prob, run, n_paths)
+    steps = 0
+    kern_ms = []
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        out = eng.assemble(mesh, prob, run, n_paths)
+        lat.append(time.perf_counter() - ts)
+        steps += out["counters"]["path_steps"]
+        kern_ms.append(out["counters"]["kernel_ms"])
+    wall = time.perf_counter() - t0
+    value = steps / wall
+    rms, iters = rms_error(eng, mesh, prob, out)
+    c = out["counters"]
+    avg_kernel_s = float(np.mean(kern_ms)) / 1e3
+    flops = FLOPS_PER_STEP.get(args.config, 302.0) * c["path_steps"]
+    achieved = flops / avg_kernel_s / 1e12
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            traffic = json.load(open(args.traffic_file)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    result = {
+        "metric": "path-steps/s at fixed RMS interface error; p50/p99 solve latency vs plain GPU",
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (manufactured solution u = e^x cos y; LUTs sampled from the analytic coefficients)",
+        "config": {"workload": "plain PDDSparse assembly, cfg2: 8x8 subdomains x 32 knots/interface, "
+                               "3437 knots (3409 MC), %d paths/knot, h=%g, FP64 paths, 512x512 LUTs, seed %d"
+                               % (n_paths, h, seed),
+                   "config_name": args.config, "knots": mesh.n_knots, "g_nnz": mesh.nnz,
+                   "paths_per_knot": n_paths, "h": h},
+        "path_steps_per_step": c["path_steps"],
+        "rms_interface_error": rms,
+        "bicgstab_iters": iters,
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "launch": {"grid_wgs": c["grid_wgs"], "wg_size": c["wg_size"], "item_paths": c["item_paths"],
+                   "sub_batch": c["sub_batch"], "work_items": c["work_items"]},
+        "device": info,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": "assemble_kernel<double, PDE_VARCOEF_LUT, persistent>",
+                     "avg_kernel_ms": 1e3 * avg_kernel_s,
+                     "note": "FP64 VALU-bound (no dense contraction); peak = MI355X FP64 vector = FP64 matrix "
+                             "78.6 TFLOP/s; achieved counts %g algorithmic flops per path-step"
+                             % FLOPS_PER_STEP.get(args.config, 302.0)},
+    }
+    if not args.no_extras:
+        extras = {}
+        for name, kw in (("plain_fp64", dict(plain=1)), ("engine_fp32", dict(precision="fp32"))):
+            r2 = default_run(h=h, seed=seed, **kw)
+            ts = time.perf_counter()
+            o2 = eng.assemble(mesh, prob, r2, n_paths)
+            dt = time.perf_counter() - ts
+            e2, _ = rms_error(eng, mesh, prob, o2)
+            extras[name] = {"path_steps_per_s": o2["counters"]["path_steps"] / dt, "seconds": dt,
+                            "kernel_ms": o2["counters"]["kernel_ms"], "rms_interface_error": e2}
+        extras["engine_fp64"] = {"path_steps_per_s": value, "seconds": wall / args.steps, "rms_interface_error": rms}
+        extras["engine_fp32_vs_plain_fp64"] = extras["engine_fp32"]["path_steps_per_s"] / extras["plain_fp64"]["path_steps_per_s"]
+        result["forms"] = extras
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(mesh, {3: "varcoef_lut", 0: "monegros", 1: "sinsin"}[prob.pde],
+                                              n_paths, h, seed)
+    eng.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -25,6 +25,26 @@ struct DevBuf {
 
 }  // namespace
 
+// Stencil tables of one mesh in device memory (kept resident across calls).
+struct DevTables {
+    uint64_t mesh_id = 0;
+    int quirk = -1;
+    int n_sten = 0;
+    StencilDesc *sten = nullptr;
+    double *entx = nullptr, *enty = nullptr, *ipsi = nullptr;
+    int8_t *ents = nullptr;
+    int32_t *e1 = nullptr, *e2 = nullptr;
+    std::vector<StencilDesc> host_sten;   // sizes for launch planning
+    size_t bytes = 0;
+    void release() {
+        void *ps[] = {sten, entx, enty, ipsi, ents, e1, e2};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        sten = nullptr; entx = enty = ipsi = nullptr; ents = nullptr; e1 = e2 = nullptr;
+        mesh_id = 0; quirk = -1; n_sten = 0; host_sten.clear(); bytes = 0;
+    }
+};
+
 struct pdds_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -34,6 +54,8 @@ struct pdds_engine {
     int n_cu = 0;
     std::string name;
     std::vector<DevBuf> bufs;   // grow-only scratch, indexed by slot
+    DevTables tables[2];        // two most recent (mesh, quirk) table sets
+    int tables_next = 0;
 };
 
 struct pdds_lut {
@@ -43,9 +65,18 @@ struct pdds_lut {
     int nx = 0, ny = 0, nf = 0, mode = 0;
 };
 
+static uint64_t next_mesh_id() {
+    static std::mutex mu;
+    static uint64_t id = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    return ++id;
+}
+
 struct pdds_mesh {
     Mesh m;
-    pdds_mesh(double a, double b, double c, double d, int n, int k, const double f[4]) : m(a, b, c, d, n, k, f) {}
+    uint64_t id;
+    pdds_mesh(double a, double b, double c, double d, int n, int k, const double f[4])
+        : m(a, b, c, d, n, k, f), id(next_mesh_id()) {}
 };
 
 namespace {
@@ -150,12 +181,38 @@ int choose_P(int64_t n, int S) {
     return (int)(q * S);
 }
 
-pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
-                         const std::vector<StencilDesc> &sten, const std::vector<double> &entx,
+template <class T>
+pdds_status dalloc_copy(pdds_engine *e, const std::vector<T> &v, T **out, size_t &bytes) {
+    size_t nb = std::max<size_t>(v.size() * sizeof(T), 8);
+    if (hipMalloc((void **)out, nb) != hipSuccess) return fail(e, PDDS_ERR_NOMEM, "device allocation failed");
+    bytes += nb;
+    if (!v.empty()) HIPCHK(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PDDS_OK;
+}
+
+// Upload host stencil tables into T (device memory owned by T).
+pdds_status build_tables(pdds_engine *e, const std::vector<StencilDesc> &sten, const std::vector<double> &entx,
                          const std::vector<double> &enty, const std::vector<int8_t> &ents,
                          const std::vector<int32_t> &e1, const std::vector<int32_t> &e2,
-                         const std::vector<double> &ipsi, const std::vector<AsmKnot> &knots, int64_t n_vals,
+                         const std::vector<double> &ipsi, DevTables &T) {
+    T.release();
+    pdds_status st;
+    if ((st = dalloc_copy(e, sten, &T.sten, T.bytes)) || (st = dalloc_copy(e, entx, &T.entx, T.bytes)) ||
+        (st = dalloc_copy(e, enty, &T.enty, T.bytes)) || (st = dalloc_copy(e, ents, &T.ents, T.bytes)) ||
+        (st = dalloc_copy(e, e1, &T.e1, T.bytes)) || (st = dalloc_copy(e, e2, &T.e2, T.bytes)) ||
+        (st = dalloc_copy(e, ipsi, &T.ipsi, T.bytes))) {
+        T.release();
+        return st;
+    }
+    T.host_sten = sten;
+    T.n_sten = (int)sten.size();
+    return PDDS_OK;
+}
+
+pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
+                         const DevTables &T, const std::vector<AsmKnot> &knots, int64_t n_vals,
                          double *G_host, double *Gvar_host, AsmOut &out) {
+    const std::vector<StencilDesc> &sten = T.host_sten;
     PdeParams pp;
     pdds_status st = make_params(e, prob, pp);
     if (st) return st;
@@ -201,33 +258,29 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     if (items.empty()) return PDDS_OK;
     const size_t shmem = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap) * sizeof(double);
     if (shmem > 160 * 1024) return fail(e, PDDS_ERR_UNSUPPORTED, "stencil too large for the LDS reduction buffers");
+    // plain mode launches one work-group per item, in chunks that bound the per-path record buffers
+    const int64_t kPlainChunk = 65536;
     int grid;
     if (persistent) {
         grid = r->grid_wgs > 0 ? r->grid_wgs : 2 * e->n_cu;
         grid = (int)std::min<int64_t>(grid, (int64_t)items.size());
     } else {
-        grid = (int)items.size();
+        grid = (int)std::min<int64_t>((int64_t)items.size(), kPlainChunk);
     }
     out.grid = grid;
     // upload
     EngineArgs a;
     std::memset(&a, 0, sizeof(a));
-    KnotDesc *dk; StencilDesc *ds; double *dex, *dey, *dip; int8_t *des; int32_t *d1, *d2; ItemDesc *dit;
+    KnotDesc *dk; ItemDesc *dit;
+    const StencilDesc *ds = T.sten;
     if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
-    if ((st = upload(e, B_STEN, sten, &ds))) return st;
-    if ((st = upload(e, B_ENTX, entx, &dex))) return st;
-    if ((st = upload(e, B_ENTY, enty, &dey))) return st;
-    if ((st = upload(e, B_ENTS, ents, &des))) return st;
-    if ((st = upload(e, B_E1, e1, &d1))) return st;
-    if ((st = upload(e, B_E2, e2, &d2))) return st;
-    if ((st = upload(e, B_IPSI, ipsi, &dip))) return st;
     if ((st = upload(e, B_ITEMS, items, &dit))) return st;
     void *p;
     if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st;
     a.stats = (ItemStat *)p;
     if ((st = dbuf(e, B_PART, (size_t)part * sizeof(double), &p))) return st;
     a.partials = (double *)p;
-    const int64_t rec_slots = persistent ? grid : (int64_t)items.size();
+    const int64_t rec_slots = grid;
     const size_t nrec = (size_t)rec_slots * (size_t)Pmax;
     if ((st = dbuf(e, B_RECX, nrec * 8, &p))) return st; a.rec_x = (double *)p;
     if ((st = dbuf(e, B_RECY, nrec * 8, &p))) return st; a.rec_y = (double *)p;
@@ -236,8 +289,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     if ((st = dbuf(e, B_RECSIDE, nrec, &p))) return st; a.rec_side = (int8_t *)p;
     if ((st = dbuf(e, B_COUNTER, 64, &p))) return st; a.work_counter = (unsigned long long *)p;
     HIPCHK(hipMemsetAsync(a.work_counter, 0, 64, e->stream));
-    a.knots = dk; a.sten = ds; a.ent_x = dex; a.ent_y = dey; a.ent_side = des; a.csr_e1 = d1; a.csr_e2 = d2;
-    a.ipsi = dip; a.items = dit; a.n_items = (int64_t)items.size();
+    a.knots = dk; a.sten = ds; a.ent_x = T.entx; a.ent_y = T.enty; a.ent_side = T.ents; a.csr_e1 = T.e1;
+    a.csr_e2 = T.e2; a.ipsi = T.ipsi; a.items = dit; a.n_items = (int64_t)items.size(); a.item_base = 0;
     a.rec_stride = Pmax;
     a.ne_cap = ne_cap; a.nnz_cap = nnz_cap;
     for (int i = 0; i < 4; ++i) a.glob[i] = glob[i];
@@ -249,10 +302,15 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     if ((st = dbuf(e, B_GVAR, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dGv = (double *)p;
     if ((st = dbuf(e, B_KSTAT, knots.size() * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
     HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    hipError_t lr = r->precision == PDDS_FP64 ? launch_assemble_f64(prob->pde, persistent, grid, wg, shmem, e->stream, a)
-                                              : launch_assemble_f32(prob->pde, persistent, grid, wg, shmem, e->stream, a);
-    if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(lr));
+    for (int64_t base = 0; base < (persistent ? 1 : (int64_t)items.size()); base += kPlainChunk) {
+        a.item_base = base;
+        int g = persistent ? grid : (int)std::min<int64_t>(kPlainChunk, (int64_t)items.size() - base);
+        hipError_t lr = r->precision == PDDS_FP64 ? launch_assemble_f64(prob->pde, persistent, g, wg, shmem, e->stream, a)
+                                                  : launch_assemble_f32(prob->pde, persistent, g, wg, shmem, e->stream, a);
+        if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(lr));
+    }
     HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    hipError_t lr;
     FinalizeArgs f;
     f.knots = dk; f.sten = ds; f.items = dit; f.stats = a.stats; f.partials = a.partials;
     f.n_knots = (int32_t)knots.size(); f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
@@ -295,6 +353,26 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
         }
         sten.push_back(d);
     }
+}
+
+// Device-resident stencil tables of (mesh, quirk): built on first use, then reused.
+pdds_status mesh_dev_tables(pdds_engine *e, const pdds_mesh *m, int quirk, DevTables **out) {
+    const int q = quirk ? 1 : 0;
+    for (DevTables &t : e->tables)
+        if (t.mesh_id == m->id && t.quirk == q) { *out = &t; return PDDS_OK; }
+    std::vector<StencilDesc> sten;
+    std::vector<double> entx, enty, ipsi;
+    std::vector<int8_t> ents;
+    std::vector<int32_t> e1, e2;
+    mesh_tables(m->m, q, sten, entx, enty, ents, e1, e2, ipsi);
+    DevTables &t = e->tables[e->tables_next];
+    e->tables_next = (e->tables_next + 1) % 2;
+    pdds_status st = build_tables(e, sten, entx, enty, ents, e1, e2, ipsi, t);
+    if (st) return st;
+    t.mesh_id = m->id;
+    t.quirk = q;
+    *out = &t;
+    return PDDS_OK;
 }
 
 void fill_counters(pdds_counters *ctr, const AsmOut &o, double total_ms) {
@@ -380,6 +458,7 @@ void pdds_engine_destroy(pdds_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto &b : e->bufs)
         if (b.p) (void)hipFree(b.p);
+    for (auto &t : e->tables) t.release();
     for (auto &ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -565,11 +644,8 @@ pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem
     PdeParams pp;
     if ((st = make_params(e, p, pp))) return st;
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
-    std::vector<StencilDesc> sten;
-    std::vector<double> entx, enty, ipsi;
-    std::vector<int8_t> ents;
-    std::vector<int32_t> e1, e2;
-    mesh_tables(M, r->quirk_last_side, sten, entx, enty, ents, e1, e2, ipsi);
+    DevTables *T;
+    if ((st = mesh_dev_tables(e, m, r->quirk_last_side, &T))) return st;
     std::vector<AsmKnot> knots;
     std::vector<int> kmap;
     for (int q = 0; q < M.n_knots(); ++q) {
@@ -586,7 +662,7 @@ pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         kmap.push_back(q);
     }
     AsmOut o;
-    st = run_assembly(e, p, r, M.glob, sten, entx, enty, ents, e1, e2, ipsi, knots, M.row_ptr.back(), G, G_var, o);
+    st = run_assembly(e, p, r, M.glob, *T, knots, M.row_ptr.back(), G, G_var, o);
     if (st) return st;
     for (size_t i = 0; i < knots.size(); ++i) { B[kmap[i]] = o.kstats[i].B; BB[kmap[i]] = o.kstats[i].BB; }
     fill_counters(ctr, o, now_ms() - t0);
@@ -614,15 +690,11 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
     if (n_paths <= 0) return fail(e, PDDS_ERR_INVALID, "n_paths must be positive");
     if (M.stencils[M.stencil_of[knot]].singular) return fail(e, PDDS_ERR_SINGULAR, "singular Psi");
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
-    std::vector<StencilDesc> sten;
-    std::vector<double> entx, enty, ipsi;
-    std::vector<int8_t> ents;
-    std::vector<int32_t> e1, e2;
-    mesh_tables(M, r->quirk_last_side, sten, entx, enty, ents, e1, e2, ipsi);
+    DevTables *T;
+    if ((st = mesh_dev_tables(e, m, r->quirk_last_side, &T))) return st;
     std::vector<AsmKnot> knots = {AsmKnot{M.x0[knot], M.y0[knot], (uint32_t)knot, M.stencil_of[knot], n_paths, 0}};
     AsmOut o;
-    st = run_assembly(e, p, r, M.glob, sten, entx, enty, ents, e1, e2, ipsi, knots,
-                      (int64_t)M.stencils[M.stencil_of[knot]].cols.size(), G, G_var, o);
+    st = run_assembly(e, p, r, M.glob, *T, knots, (int64_t)M.stencils[M.stencil_of[knot]].cols.size(), G, G_var, o);
     if (st) return st;
     *B = o.kstats[0].B;
     *BB = o.kstats[0].BB;
@@ -651,6 +723,9 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
     std::vector<double> entx, enty, ipsi;
     std::vector<int8_t> ents;
     std::vector<int32_t> e1, e2;
+    DevTables T;
+    if ((st = build_tables(e, sten, entx, enty, ents, e1, e2, ipsi, T))) return st;
+    struct Rel { DevTables &t; ~Rel() { t.release(); } } rel{T};
     std::vector<AsmKnot> knots;
     std::vector<int> kmap;
     for (int i = 0; i < n; ++i) {
@@ -668,7 +743,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
         kmap.push_back(i);
     }
     AsmOut o;
-    st = run_assembly(e, p, r, glob, sten, entx, enty, ents, e1, e2, ipsi, knots, 0, nullptr, nullptr, o);
+    st = run_assembly(e, p, r, glob, T, knots, 0, nullptr, nullptr, o);
     if (st) return st;
     for (size_t i = 0; i < knots.size(); ++i) {
         u[kmap[i]] = o.kstats[i].B;

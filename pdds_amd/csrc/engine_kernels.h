@@ -146,12 +146,16 @@ __global__ void __launch_bounds__(256) assemble_kernel(EngineArgs a) {
     double *accS = smem + kNBG * a.ne_cap;
     double *accQ = accS + a.nnz_cap;
 
-    int64_t item = PERSIST ? -1 : (int64_t)blockIdx.x;
+    int64_t item = PERSIST ? -1 : a.item_base + (int64_t)blockIdx.x;
     while (true) {
         if (PERSIST) {
             if (tid == 0) s_item = (int64_t)atomicAdd((unsigned long long *)a.work_counter, 1ULL);
             __syncthreads();
-            item = s_item;
+            // the item is the same for the whole work-group: keep it (and every descriptor
+            // loaded through it) in scalar registers
+            const int64_t v = s_item;
+            item = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                             (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v));
         }
         if (item >= a.n_items) break;
         const ItemDesc it = a.items[item];

@@ -62,6 +62,7 @@ struct EngineArgs {
     const double *ipsi;
     const ItemDesc *items;
     int64_t n_items;
+    int64_t item_base;      // non-persistent launches: item = item_base + blockIdx.x
     ItemStat *stats;
     double *partials;
     double *rec_x, *rec_y, *rec_Y, *rec_score;

@@ -129,6 +129,20 @@ static void orc_monegros(double x, double y, double *u, double *gx, double *gy, 
     *lap = dxx + dyy;
 }
 
+/* Uniform cubic B-spline weights of taps -1..2 (Horner forms with fmaf) */
+static void orc_bspline_wf(float f, float w[4]) {
+    const float s6 = 1.0f / 6.0f;
+    const float f2 = f * f, om = 1.0f - f;
+    w[0] = (om * om * om) * s6;
+    w[1] = fmaf(fmaf(3.0f, f, -6.0f), f2, 4.0f) * s6;
+    w[2] = fmaf(fmaf(fmaf(-3.0f, f, 3.0f), f, 3.0f), f, 1.0f) * s6;
+    w[3] = (f2 * f) * s6;
+}
+
+/* Bicubic B-spline lookup: the coordinate in FP64, the 16-tap weighted sum of the FP32
+   coefficients in FP32 with fma -- the arithmetic of the reference's FP32 texture path
+   (Integrators/cubicTex2D.cu:64-87, Integrators/bspline_kernel.cu:52-60).  lut_nf is the
+   texel stride in floats. */
 double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
     const int nx = pb->lut_nx, ny = pb->lut_ny, nf = pb->lut_nf;
     double t = (x - pb->lut_ox) * pb->lut_sx;
@@ -136,34 +150,21 @@ double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
     t = fmin(fmax(t, -2.0), (double)nx + 1.0);
     v = fmin(fmax(v, -2.0), (double)ny + 1.0);
     double fi = floor(t), fj = floor(v);
-    double fx = t - fi, fy = v - fj;
     int i = (int)fi, j = (int)fj;
-    double wx[4], wy[4];
-    {
-        double f2 = fx * fx, f3 = f2 * fx, om = 1.0 - fx;
-        wx[0] = (om * om * om) * (1.0 / 6.0);
-        wx[1] = (3.0 * f3 - 6.0 * f2 + 4.0) * (1.0 / 6.0);
-        wx[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * fx + 1.0) * (1.0 / 6.0);
-        wx[3] = f3 * (1.0 / 6.0);
-    }
-    {
-        double f2 = fy * fy, f3 = f2 * fy, om = 1.0 - fy;
-        wy[0] = (om * om * om) * (1.0 / 6.0);
-        wy[1] = (3.0 * f3 - 6.0 * f2 + 4.0) * (1.0 / 6.0);
-        wy[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * fy + 1.0) * (1.0 / 6.0);
-        wy[3] = f3 * (1.0 / 6.0);
-    }
-    double acc = 0.0;
+    float wx[4], wy[4];
+    orc_bspline_wf((float)(t - fi), wx);
+    orc_bspline_wf((float)(v - fj), wy);
+    float acc = 0.0f;
     for (int b = 0; b < 4; ++b) {
         int jj = j - 1 + b; jj = jj < 0 ? 0 : (jj > ny - 1 ? ny - 1 : jj);
-        double row = 0.0;
+        float row = 0.0f;
         for (int a = 0; a < 4; ++a) {
             int ii = i - 1 + a; ii = ii < 0 ? 0 : (ii > nx - 1 ? nx - 1 : ii);
-            row = row + wx[a] * (double)pb->lut[((int64_t)jj * nx + ii) * nf + field];
+            row = fmaf(wx[a], pb->lut[((int64_t)jj * nx + ii) * nf + field], row);
         }
-        acc = acc + wy[b] * row;
+        acc = fmaf(wy[b], row, acc);
     }
-    return acc;
+    return (double)acc;
 }
 
 /* Coefficients at (x,y): sigma (row-major s00 s01 s10 s11), b, c, f, grad u, g, u */
@@ -338,6 +339,12 @@ static inline double orc_rbf(double px, double py, double x, double y, double c)
     return orc_exp(-(r * r) / (c * c));
 }
 
+/* The same kernel as the GPU engine evaluates it: squared distance times a precomputed 1/c^2 */
+static inline double orc_rbf_ic2(double px, double py, double x, double y, double inv_c2) {
+    double dx = px - x, dy = py - y;
+    return orc_exp(-(dx * dx + dy * dy) * inv_c2);
+}
+
 /* CSR row of a stencil: sorted unique columns; for each side entry its column slot. */
 static int orc_row_pattern(const int msz[4], const int *idx, int *cols, int *slot) {
     int tot = msz[0] + msz[1] + msz[2] + msz[3];
@@ -436,6 +443,8 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
         use[e] = 1;
         if (quirk) for (int e2 = e + 1; e2 < tot; ++e2) if (slot[e2] == slot[e]) { use[e] = 0; break; }
     }
+    double ic2[4];
+    for (int s = 0; s < 4; ++s) ic2[s] = 1.0 / (cshape[s] * cshape[s]);
     double *Phi = (double *)calloc((size_t)tot + 1, sizeof(double));
     double *Gb = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *isum = (double *)calloc((size_t)nnz + 1, sizeof(double));
@@ -469,7 +478,7 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
                         cnt[side]++;
                         for (int j = 0; j < msz[side]; ++j)
                             Phi[off[side] + j] = Phi[off[side] + j] +
-                                Y[n] * orc_rbf(px[off[side] + j], py[off[side] + j], xt[n], yt[n], cshape[side]);
+                                Y[n] * orc_rbf_ic2(px[off[side] + j], py[off[side] + j], xt[n], yt[n], ic2[side]);
                     }
                     sc = Z[n] + xi[n];
                 }

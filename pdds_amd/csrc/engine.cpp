@@ -56,6 +56,8 @@ struct pdds_engine {
     std::vector<DevBuf> bufs;   // grow-only scratch, indexed by slot
     DevTables tables[2];        // two most recent (mesh, quirk) table sets
     int tables_next = 0;
+    int64_t record_budget = (int64_t)24 << 30;   // bytes of exit records per assembly batch
+    int path_wgs_per_cu = 8;                      // persistent path-kernel work-groups per CU
 };
 
 struct pdds_lut {
@@ -83,7 +85,7 @@ namespace {
 
 enum BufSlot {
     B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_RECX, B_RECY,
-    B_RECYY, B_RECS, B_RECSIDE, B_COUNTER, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
+    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
     B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
 };
 
@@ -174,10 +176,12 @@ struct AsmOut {
     int grid = 0, wg = 0, P = 0, S = 0;
 };
 
+// Paths per work item of the reduction: a multiple of S, about N/16, at most 8192.
 int choose_P(int64_t n, int S) {
     int64_t q = (n + 16 * (int64_t)S - 1) / (16 * (int64_t)S);
     if (q < 1) q = 1;
-    if (q > 128) q = 128;
+    if (q * S > 8192) q = 8192 / S;
+    if (q < 1) q = 1;
     return (int)(q * S);
 }
 
@@ -220,112 +224,135 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     const int wg = r->wg_size > 0 ? r->wg_size : 256;
     const bool persistent = r->plain == 0;
     const int64_t max_steps = r->max_steps > 0 ? r->max_steps : (int64_t)1 << 26;
-    if (!persistent && wg % S != 0) return fail(e, PDDS_ERR_INVALID, "plain mode needs wg_size to be a multiple of sub_batch");
     if (r->item_paths > 0 && r->item_paths % S != 0)
         return fail(e, PDDS_ERR_INVALID, "item_paths must be a multiple of sub_batch");
-    // items
-    std::vector<KnotDesc> kd(knots.size());
-    std::vector<ItemDesc> items;
-    int Pmax = 0;
-    int64_t part = 0;
-    int ne_cap = 1, nnz_cap = 1;
-    for (size_t q = 0; q < knots.size(); ++q) {
-        const AsmKnot &k = knots[q];
-        const StencilDesc &sd = sten[k.stencil];
-        const int P = !persistent ? wg : (r->item_paths > 0 ? r->item_paths : choose_P(k.n, S));
-        Pmax = std::max(Pmax, P);
-        ne_cap = std::max(ne_cap, sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3]);
-        nnz_cap = std::max(nnz_cap, sd.csr_nnz);
-        kd[q].x0 = k.x0; kd[q].y0 = k.y0; kd[q].knot_id = k.id; kd[q].stencil = k.stencil;
-        kd[q].n_paths = k.n; kd[q].out_off = k.out_off;
-        kd[q].item_begin = (int64_t)items.size();
-        for (int64_t p0 = 0; p0 < k.n; p0 += P) {
-            ItemDesc it;
-            it.knot = (int32_t)q;
-            it.n = (int32_t)std::min<int64_t>(P, k.n - p0);
-            it.p_begin = p0;
-            it.part_off = part;
-            part += 2 * (int64_t)sd.csr_nnz;
-            items.push_back(it);
-        }
-        kd[q].item_count = (int64_t)items.size() - kd[q].item_begin;
-    }
-    out.items = (int64_t)items.size();
+    if (max_steps >= kCappedBit) return fail(e, PDDS_ERR_INVALID, "max_steps must be below 2^30");
     out.S = S;
     out.wg = wg;
-    out.P = Pmax;
     out.kstats.assign(knots.size(), KnotStat{});
-    if (items.empty()) return PDDS_OK;
-    const size_t shmem = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap) * sizeof(double);
-    if (shmem > 160 * 1024) return fail(e, PDDS_ERR_UNSUPPORTED, "stencil too large for the LDS reduction buffers");
-    // plain mode launches one work-group per item, in chunks that bound the per-path record buffers
-    const int64_t kPlainChunk = 65536;
-    int grid;
-    if (persistent) {
-        grid = r->grid_wgs > 0 ? r->grid_wgs : 2 * e->n_cu;
-        grid = (int)std::min<int64_t>(grid, (int64_t)items.size());
-    } else {
-        grid = (int)std::min<int64_t>((int64_t)items.size(), kPlainChunk);
-    }
-    out.grid = grid;
-    // upload
-    EngineArgs a;
-    std::memset(&a, 0, sizeof(a));
-    KnotDesc *dk; ItemDesc *dit;
-    const StencilDesc *ds = T.sten;
-    if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
-    if ((st = upload(e, B_ITEMS, items, &dit))) return st;
+    if (knots.empty()) return PDDS_OK;
+    // knots are processed in batches whose exit records fit the record budget
+    const int64_t rec_bytes = 4 * 8 + 1 + 4;
+    const int64_t budget = e->record_budget / rec_bytes;
     void *p;
-    if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st;
-    a.stats = (ItemStat *)p;
-    if ((st = dbuf(e, B_PART, (size_t)part * sizeof(double), &p))) return st;
-    a.partials = (double *)p;
-    const int64_t rec_slots = grid;
-    const size_t nrec = (size_t)rec_slots * (size_t)Pmax;
-    if ((st = dbuf(e, B_RECX, nrec * 8, &p))) return st; a.rec_x = (double *)p;
-    if ((st = dbuf(e, B_RECY, nrec * 8, &p))) return st; a.rec_y = (double *)p;
-    if ((st = dbuf(e, B_RECYY, nrec * 8, &p))) return st; a.rec_Y = (double *)p;
-    if ((st = dbuf(e, B_RECS, nrec * 8, &p))) return st; a.rec_score = (double *)p;
-    if ((st = dbuf(e, B_RECSIDE, nrec, &p))) return st; a.rec_side = (int8_t *)p;
-    if ((st = dbuf(e, B_COUNTER, 64, &p))) return st; a.work_counter = (unsigned long long *)p;
-    HIPCHK(hipMemsetAsync(a.work_counter, 0, 64, e->stream));
-    a.knots = dk; a.sten = ds; a.ent_x = T.entx; a.ent_y = T.enty; a.ent_side = T.ents; a.csr_e1 = T.e1;
-    a.csr_e2 = T.e2; a.ipsi = T.ipsi; a.items = dit; a.n_items = (int64_t)items.size(); a.item_base = 0;
-    a.rec_stride = Pmax;
-    a.ne_cap = ne_cap; a.nnz_cap = nnz_cap;
-    for (int i = 0; i < 4; ++i) a.glob[i] = glob[i];
-    a.h = r->h; a.sqrth = std::sqrt(r->h); a.beta = r->beta;
-    a.seed = r->seed; a.stream = r->stream; a.max_steps = max_steps; a.S = S;
-    a.pp = pp;
-    double *dG, *dGv; KnotStat *dks;
+    double *dG, *dGv;
     if ((st = dbuf(e, B_G, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dG = (double *)p;
     if ((st = dbuf(e, B_GVAR, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dGv = (double *)p;
-    if ((st = dbuf(e, B_KSTAT, knots.size() * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
-    HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    for (int64_t base = 0; base < (persistent ? 1 : (int64_t)items.size()); base += kPlainChunk) {
-        a.item_base = base;
-        int g = persistent ? grid : (int)std::min<int64_t>(kPlainChunk, (int64_t)items.size() - base);
-        hipError_t lr = r->precision == PDDS_FP64 ? launch_assemble_f64(prob->pde, persistent, g, wg, shmem, e->stream, a)
-                                                  : launch_assemble_f32(prob->pde, persistent, g, wg, shmem, e->stream, a);
-        if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(lr));
+    double kernel_ms = 0.0, reduce_ms = 0.0;
+    int grid_used = 0, Pmax_all = 0;
+    int64_t items_all = 0;
+    size_t k0 = 0;
+    while (k0 < knots.size()) {
+        // batch [k0, k1)
+        size_t k1 = k0;
+        int64_t tot = 0;
+        while (k1 < knots.size() && (k1 == k0 || tot + knots[k1].n <= budget)) tot += knots[k1++].n;
+        const int nk = (int)(k1 - k0);
+        std::vector<KnotDesc> kd(nk);
+        std::vector<ItemDesc> items;
+        int64_t part = 0, off = 0;
+        int ne_cap = 1, nnz_cap = 1;
+        for (int q = 0; q < nk; ++q) {
+            const AsmKnot &k = knots[k0 + q];
+            const StencilDesc &sd = sten[k.stencil];
+            const int P = r->item_paths > 0 ? r->item_paths : choose_P(k.n, S);
+            Pmax_all = std::max(Pmax_all, P);
+            ne_cap = std::max(ne_cap, sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3]);
+            nnz_cap = std::max(nnz_cap, sd.csr_nnz);
+            kd[q].x0 = k.x0; kd[q].y0 = k.y0; kd[q].knot_id = k.id; kd[q].stencil = k.stencil;
+            kd[q].n_paths = k.n; kd[q].path_off = off; kd[q].out_off = k.out_off;
+            kd[q].item_begin = (int64_t)items.size();
+            for (int64_t p0 = 0; p0 < k.n; p0 += P) {
+                ItemDesc it;
+                it.knot = q;
+                it.n = (int32_t)std::min<int64_t>(P, k.n - p0);
+                it.p_begin = p0;
+                it.part_off = part;
+                part += 2 * (int64_t)sd.csr_nnz;
+                items.push_back(it);
+            }
+            kd[q].item_count = (int64_t)items.size() - kd[q].item_begin;
+            off += k.n;
+        }
+        items_all += (int64_t)items.size();
+        const size_t shmem = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap) * sizeof(double);
+        if (shmem > 160 * 1024) return fail(e, PDDS_ERR_UNSUPPORTED, "stencil too large for the LDS reduction buffers");
+        KnotDesc *dk; ItemDesc *dit;
+        if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
+        if ((st = upload(e, B_ITEMS, items, &dit))) return st;
+        Records rec;
+        if ((st = dbuf(e, B_RECX, (size_t)tot * 8, &p))) return st; rec.x = (double *)p;
+        if ((st = dbuf(e, B_RECY, (size_t)tot * 8, &p))) return st; rec.y = (double *)p;
+        if ((st = dbuf(e, B_RECYY, (size_t)tot * 8, &p))) return st; rec.Y = (double *)p;
+        if ((st = dbuf(e, B_RECS, (size_t)tot * 8, &p))) return st; rec.score = (double *)p;
+        if ((st = dbuf(e, B_RECSIDE, (size_t)tot, &p))) return st; rec.side = (int8_t *)p;
+        if ((st = dbuf(e, B_RECSTEPS, (size_t)tot * 4, &p))) return st; rec.steps = (int32_t *)p;
+        ItemStat *dstats; double *dpart; KnotStat *dks; unsigned long long *dctr;
+        if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st; dstats = (ItemStat *)p;
+        if ((st = dbuf(e, B_PART, (size_t)std::max<int64_t>(part, 1) * sizeof(double), &p))) return st; dpart = (double *)p;
+        if ((st = dbuf(e, B_KSTAT, (size_t)nk * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
+        if ((st = dbuf(e, B_COUNTER, 64, &p))) return st; dctr = (unsigned long long *)p;
+        HIPCHK(hipMemsetAsync(dctr, 0, 64, e->stream));
+        // path kernel
+        PathKernelArgs pa;
+        std::memset(&pa, 0, sizeof(pa));
+        pa.knots = dk; pa.n_knots = nk; pa.sten = T.sten; pa.total = tot; pa.base = 0; pa.counter = dctr; pa.rec = rec;
+        for (int i = 0; i < 4; ++i) pa.glob[i] = glob[i];
+        pa.h = r->h; pa.sqrth = std::sqrt(r->h); pa.beta = r->beta;
+        pa.seed = r->seed; pa.stream = r->stream; pa.max_steps = max_steps; pa.pp = pp;
+        int grid;
+        if (persistent) {
+            grid = r->grid_wgs > 0 ? r->grid_wgs : e->path_wgs_per_cu * e->n_cu;
+            grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (tot + wg - 1) / wg));
+        } else {
+            grid = (int)std::min<int64_t>((tot + wg - 1) / wg, (int64_t)1 << 30);
+        }
+        grid_used = std::max(grid_used, grid);
+        HIPCHK(hipEventRecord(e->ev[0], e->stream));
+        const int64_t per_launch = persistent ? tot : (int64_t)grid * wg;
+        for (int64_t base = 0; base < tot; base += per_launch) {
+            pa.base = base;
+            const int g = persistent ? grid : (int)std::min<int64_t>(grid, (tot - base + wg - 1) / wg);
+            hipError_t lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, g, wg, e->stream, pa)
+                                                      : launch_path_f32(prob->pde, persistent, g, wg, e->stream, pa);
+            if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("path launch: ") + hipGetErrorString(lr));
+        }
+        HIPCHK(hipEventRecord(e->ev[1], e->stream));
+        // reduction + finalize
+        ReduceArgs ra;
+        std::memset(&ra, 0, sizeof(ra));
+        ra.knots = dk; ra.sten = T.sten; ra.ent_x = T.entx; ra.ent_y = T.enty; ra.ent_side = T.ents;
+        ra.csr_e1 = T.e1; ra.csr_e2 = T.e2; ra.ipsi = T.ipsi; ra.items = dit; ra.rec = rec;
+        ra.stats = dstats; ra.partials = dpart; ra.ne_cap = ne_cap; ra.nnz_cap = nnz_cap; ra.S = S;
+        const int64_t nit = (int64_t)items.size();
+        for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
+            ra.item_base = base;
+            hipError_t lr = launch_reduce((int)std::min<int64_t>(nit - base, (int64_t)1 << 30), 256, shmem, e->stream, ra);
+            if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("reduce launch: ") + hipGetErrorString(lr));
+        }
+        FinalizeArgs f;
+        f.knots = dk; f.sten = T.sten; f.items = dit; f.stats = dstats; f.partials = dpart;
+        f.n_knots = nk; f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
+        hipError_t lr = launch_finalize(nk, 256, e->stream, f);
+        if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
+        HIPCHK(hipEventRecord(e->ev[2], e->stream));
+        HIPCHK(hipMemcpyAsync(out.kstats.data() + k0, dks, (size_t)nk * sizeof(KnotStat), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        float ms1 = 0, ms2 = 0;
+        HIPCHK(hipEventElapsedTime(&ms1, e->ev[0], e->ev[1]));
+        HIPCHK(hipEventElapsedTime(&ms2, e->ev[1], e->ev[2]));
+        kernel_ms += ms1;
+        reduce_ms += ms2;
+        k0 = k1;
     }
-    HIPCHK(hipEventRecord(e->ev[1], e->stream));
-    hipError_t lr;
-    FinalizeArgs f;
-    f.knots = dk; f.sten = ds; f.items = dit; f.stats = a.stats; f.partials = a.partials;
-    f.n_knots = (int32_t)knots.size(); f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
-    lr = launch_finalize((int)knots.size(), 256, e->stream, f);
-    if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
-    HIPCHK(hipEventRecord(e->ev[2], e->stream));
     if (n_vals > 0 && G_host) HIPCHK(hipMemcpyAsync(G_host, dG, (size_t)n_vals * 8, hipMemcpyDeviceToHost, e->stream));
     if (n_vals > 0 && Gvar_host) HIPCHK(hipMemcpyAsync(Gvar_host, dGv, (size_t)n_vals * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(out.kstats.data(), dks, knots.size() * sizeof(KnotStat), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, e->ev[0], e->ev[1]));
-    HIPCHK(hipEventElapsedTime(&ms2, e->ev[1], e->ev[2]));
-    out.kernel_ms = ms1;
-    out.reduce_ms = ms2;
+    out.kernel_ms = kernel_ms;
+    out.reduce_ms = reduce_ms;
+    out.items = items_all;
+    out.grid = grid_used;
+    out.P = Pmax_all;
     return PDDS_OK;
 }
 
@@ -336,7 +363,11 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
     for (const Stencil &s : m.stencils) {
         StencilDesc d;
         std::memset(&d, 0, sizeof(d));
-        for (int i = 0; i < 4; ++i) { d.rect[i] = s.rect[i]; d.cshape[i] = s.cshape[i]; d.msz[i] = (int)s.idx[i].size(); }
+        for (int i = 0; i < 4; ++i) {
+            d.rect[i] = s.rect[i];
+            d.inv_c2[i] = 1.0 / (s.cshape[i] * s.cshape[i]);
+            d.msz[i] = (int)s.idx[i].size();
+        }
         d.ent_off = (int32_t)entx.size();
         d.csr_nnz = (int32_t)s.cols.size();
         d.csr_off = (int32_t)e1.size();
@@ -501,14 +532,19 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
         v.sx = (double)nx / (x1 - x0); v.sy = (double)ny / (y1 - y0);
         v.ox = x0 + 0.5 / v.sx; v.oy = y0 + 0.5 / v.sy;
     }
-    v.nx = NX; v.ny = NY; v.nf = nf;
+    // texels padded to 4 or 8 floats so that a tap is one or two 16-byte vector loads
+    const int stride = nf <= 4 ? 4 : 8;
+    std::vector<float> padded((size_t)NX * NY * stride, 0.0f);
+    for (size_t t = 0; t < (size_t)NX * NY; ++t)
+        for (int f = 0; f < nf; ++f) padded[t * stride + f] = coef[t * nf + f];
+    v.nx = NX; v.ny = NY; v.nf = nf; v.stride = stride;
     pdds_lut *l = new pdds_lut();
     l->e = e;
-    if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, coef.size() * sizeof(float)) != hipSuccess) {
+    if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, padded.size() * sizeof(float)) != hipSuccess) {
         delete l;
         return fail(e, PDDS_ERR_NOMEM, "LUT allocation failed");
     }
-    if (hipMemcpy(l->d, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(l->d, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(l->d);
         delete l;
         return fail(e, PDDS_ERR_DEVICE, "LUT upload failed");
@@ -717,7 +753,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     StencilDesc d;
     std::memset(&d, 0, sizeof(d));
-    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.cshape[i] = 1.0; }
+    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.inv_c2[i] = 1.0; }
     d.gmask = 15;
     std::vector<StencilDesc> sten = {d};
     std::vector<double> entx, enty, ipsi;

@@ -6,22 +6,21 @@
 namespace pdds {
 
 template <int PDE>
-static hipError_t asm_f32(bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a) {
+static hipError_t path_f32(bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     if (persistent)
-        hipLaunchKernelGGL((assemble_kernel<float, PDE, true>), dim3(grid), dim3(wg), shmem, s, a);
+        hipLaunchKernelGGL((path_kernel<float, PDE, true>), dim3(grid), dim3(wg), 0, s, a);
     else
-        hipLaunchKernelGGL((assemble_kernel<float, PDE, false>), dim3(grid), dim3(wg), shmem, s, a);
+        hipLaunchKernelGGL((path_kernel<float, PDE, false>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_f32(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s,
-                               const EngineArgs &a) {
+hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     switch (pde) {
-        case PDE_MONEGROS: return asm_f32<PDE_MONEGROS>(persistent, grid, wg, shmem, s, a);
-        case PDE_SINSIN: return asm_f32<PDE_SINSIN>(persistent, grid, wg, shmem, s, a);
-        case PDE_VARCOEF: return asm_f32<PDE_VARCOEF>(persistent, grid, wg, shmem, s, a);
-        case PDE_VARCOEF_LUT: return asm_f32<PDE_VARCOEF_LUT>(persistent, grid, wg, shmem, s, a);
-        case PDE_HELMHOLTZ: return asm_f32<PDE_HELMHOLTZ>(persistent, grid, wg, shmem, s, a);
+        case PDE_MONEGROS: return path_f32<PDE_MONEGROS>(persistent, grid, wg, s, a);
+        case PDE_SINSIN: return path_f32<PDE_SINSIN>(persistent, grid, wg, s, a);
+        case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(persistent, grid, wg, s, a);
+        case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(persistent, grid, wg, s, a);
+        case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(persistent, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }

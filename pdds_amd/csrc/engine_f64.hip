@@ -1,28 +1,28 @@
-// engine_f64.hip -- FP64 instantiations of the path kernels.  Built with
-// -ffp-contract=off so that the only fused multiply-adds are the explicit
-// fma() calls of pdds_math.h: every FP64 path is then bit-identical to the
+// engine_f64.hip -- FP64 instantiations of the path kernels, and the reduction,
+// finalize and LUT-evaluation kernels (shared by both path precisions).  Built
+// with -ffp-contract=off so that the only fused multiply-adds are the explicit
+// fma() calls: every FP64 path and every reduction is then bit-identical to the
 // FP64 CPU restatement on the same Philox stream.
 #include "engine_kernels.h"
 
 namespace pdds {
 
 template <int PDE>
-static hipError_t asm_f64(bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a) {
+static hipError_t path_f64(bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     if (persistent)
-        hipLaunchKernelGGL((assemble_kernel<double, PDE, true>), dim3(grid), dim3(wg), shmem, s, a);
+        hipLaunchKernelGGL((path_kernel<double, PDE, true>), dim3(grid), dim3(wg), 0, s, a);
     else
-        hipLaunchKernelGGL((assemble_kernel<double, PDE, false>), dim3(grid), dim3(wg), shmem, s, a);
+        hipLaunchKernelGGL((path_kernel<double, PDE, false>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_f64(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s,
-                               const EngineArgs &a) {
+hipError_t launch_path_f64(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     switch (pde) {
-        case PDE_MONEGROS: return asm_f64<PDE_MONEGROS>(persistent, grid, wg, shmem, s, a);
-        case PDE_SINSIN: return asm_f64<PDE_SINSIN>(persistent, grid, wg, shmem, s, a);
-        case PDE_VARCOEF: return asm_f64<PDE_VARCOEF>(persistent, grid, wg, shmem, s, a);
-        case PDE_VARCOEF_LUT: return asm_f64<PDE_VARCOEF_LUT>(persistent, grid, wg, shmem, s, a);
-        case PDE_HELMHOLTZ: return asm_f64<PDE_HELMHOLTZ>(persistent, grid, wg, shmem, s, a);
+        case PDE_MONEGROS: return path_f64<PDE_MONEGROS>(persistent, grid, wg, s, a);
+        case PDE_SINSIN: return path_f64<PDE_SINSIN>(persistent, grid, wg, s, a);
+        case PDE_VARCOEF: return path_f64<PDE_VARCOEF>(persistent, grid, wg, s, a);
+        case PDE_VARCOEF_LUT: return path_f64<PDE_VARCOEF_LUT>(persistent, grid, wg, s, a);
+        case PDE_HELMHOLTZ: return path_f64<PDE_HELMHOLTZ>(persistent, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }
@@ -42,6 +42,138 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
         case PDE_HELMHOLTZ: return paths_f64<PDE_HELMHOLTZ>(grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
+}
+
+// One work-group per work item: per-side RBF moments per sub-batch, G_beta = -Psi^-T Phi per
+// CSR column, sums in fixed order (sub-batch -> item), B sums per sub-batch.
+__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
+    extern __shared__ double smem[];
+    __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
+    __shared__ unsigned long long s_steps;
+    __shared__ unsigned int s_cnt[8];
+    const int tid = threadIdx.x;
+    const int WG = blockDim.x;
+    const int64_t item = a.item_base + (int64_t)blockIdx.x;
+    const ItemDesc it = a.items[item];
+    const KnotDesc kd = a.knots[it.knot];
+    const StencilDesc &sd = a.sten[kd.stencil];
+    const int64_t g0 = kd.path_off + it.p_begin;
+    const double *rx = a.rec.x + g0, *ry = a.rec.y + g0, *rY = a.rec.Y + g0, *rs = a.rec.score + g0;
+    const int8_t *rside = a.rec.side + g0;
+    const int32_t *rst = a.rec.steps + g0;
+    double *Phi = smem;
+    double *accS = smem + kNBG * a.ne_cap;
+    double *accQ = accS + a.nnz_cap;
+    if (tid == 0) {
+        s_steps = 0ULL;
+        for (int q = 0; q < 8; ++q) s_cnt[q] = 0u;
+    }
+    __syncthreads();
+    {
+        unsigned long long my = 0ULL;
+        unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int p = tid; p < it.n; p += WG) {
+            const int32_t st = rst[p];
+            my += (unsigned long long)(st & ~kCappedBit);
+            if (st & kCappedBit) cnt[6]++;
+            const int sd0 = rside[p];
+            cnt[sd0 < 0 ? 5 : sd0]++;
+        }
+        atomicAdd(&s_steps, my);
+        for (int q = 0; q < 7; ++q)
+            if (cnt[q]) atomicAdd(&s_cnt[q], cnt[q]);
+    }
+    const int ne = sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3];
+    const int nnz = sd.csr_nnz;
+    const int S = a.S;
+    const int nsb = (it.n + S - 1) / S;
+    int eoff[5];
+    eoff[0] = 0;
+    for (int q = 0; q < 4; ++q) eoff[q + 1] = eoff[q] + sd.msz[q];
+    for (int c = tid; c < nnz; c += WG) { accS[c] = 0.0; accQ[c] = 0.0; }
+    double Bs = 0.0, BBs = 0.0;
+    for (int g = 0; g < nsb; g += kNBG) {
+        const int nb = (nsb - g) < kNBG ? (nsb - g) : kNBG;
+        for (int e = tid; e < ne; e += WG) {
+            const int se = a.ent_side[sd.ent_off + e];
+            const double pxe = a.ent_x[sd.ent_off + e], pye = a.ent_y[sd.ent_off + e];
+            const double ic2 = sd.inv_c2[se];
+            for (int b = 0; b < nb; ++b) {
+                const int p0 = (g + b) * S;
+                const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
+                double acc = 0.0;
+                for (int p = p0; p < p1; ++p)
+                    if (rside[p] == se) acc = acc + rY[p] * rbf(pxe, pye, rx[p], ry[p], ic2);
+                Phi[b * a.ne_cap + e] = acc;
+            }
+        }
+        for (int b = tid; b < nb; b += WG) {
+            const int p0 = (g + b) * S;
+            const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
+            double bs = 0.0, bb = 0.0;
+            for (int p = p0; p < p1; ++p) { const double v = rs[p]; bs = bs + v; bb = bb + v * v; }
+            s_bsb[b] = bs;
+            s_bbsb[b] = bb;
+        }
+        __syncthreads();
+        for (int c = tid; c < nnz; c += WG) {
+            double G[kNBG];
+#pragma unroll
+            for (int b = 0; b < kNBG; ++b) G[b] = 0.0;
+            const int ec[2] = {a.csr_e1[sd.csr_off + c], a.csr_e2[sd.csr_off + c]};
+            for (int q = 0; q < 2; ++q) {
+                const int e = ec[q];
+                if (e < 0) continue;
+                const int se = a.ent_side[sd.ent_off + e];
+                const int m = sd.msz[se];
+                const int el = e - eoff[se];
+                const double *A = a.ipsi + sd.ipsi_off[se];
+                double H[kNBG];
+#pragma unroll
+                for (int b = 0; b < kNBG; ++b) H[b] = 0.0;
+                for (int j = 0; j < m; ++j) {
+                    const double aj = A[(int64_t)j * m + el];
+                    const double *ph = Phi + eoff[se] + j;
+#pragma unroll
+                    for (int b = 0; b < kNBG; ++b)
+                        if (b < nb) H[b] = H[b] + aj * ph[b * a.ne_cap];
+                }
+#pragma unroll
+                for (int b = 0; b < kNBG; ++b) G[b] = G[b] + (-H[b]);
+            }
+            double sum = accS[c], q2 = accQ[c];
+            for (int b = 0; b < nb; ++b) {
+                const int p0 = (g + b) * S;
+                const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
+                sum = sum + G[b];
+                q2 = q2 + (G[b] * G[b]) / (double)(p1 - p0);
+            }
+            accS[c] = sum;
+            accQ[c] = q2;
+        }
+        if (tid == 0)
+            for (int b = 0; b < nb; ++b) { Bs = Bs + s_bsb[b]; BBs = BBs + s_bbsb[b]; }
+        __syncthreads();
+    }
+    for (int c = tid; c < nnz; c += WG) {
+        a.partials[it.part_off + c] = accS[c];
+        a.partials[it.part_off + nnz + c] = accQ[c];
+    }
+    if (tid == 0) {
+        ItemStat st;
+        st.Bs = Bs;
+        st.BBs = BBs;
+        st.steps = s_steps;
+        for (int q = 0; q < 6; ++q) st.cnt[q] = s_cnt[q];
+        st.capped = s_cnt[6];
+        st.pad = 0;
+        a.stats[item] = st;
+    }
+}
+
+hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(grid), dim3(wg), shmem, s, a);
+    return hipGetLastError();
 }
 
 // One block per knot: sum the knot's items in item order.
@@ -94,16 +226,8 @@ __global__ void lut_eval_kernel(LutView L, int field, int64_t n, const double *x
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double v[8];
-    switch (L.nf) {
-        case 1: lut_eval<double, 1>(L, x[i], y[i], v); break;
-        case 2: lut_eval<double, 2>(L, x[i], y[i], v); break;
-        case 3: lut_eval<double, 3>(L, x[i], y[i], v); break;
-        case 4: lut_eval<double, 4>(L, x[i], y[i], v); break;
-        case 5: lut_eval<double, 5>(L, x[i], y[i], v); break;
-        case 6: lut_eval<double, 6>(L, x[i], y[i], v); break;
-        case 7: lut_eval<double, 7>(L, x[i], y[i], v); break;
-        default: lut_eval<double, 8>(L, x[i], y[i], v); break;
-    }
+    if (L.stride == 8) lut_eval<double, 8>(L, x[i], y[i], v);
+    else lut_eval<double, 4>(L, x[i], y[i], v);
     out[i] = v[field];
 }
 

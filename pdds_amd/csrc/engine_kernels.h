@@ -2,21 +2,23 @@
 // and on the PDE functor).  Included by engine_f64.hip (compiled with
 // -ffp-contract=off, bit-exact with the FP64 oracle) and engine_f32.hip.
 //
-// Work decomposition: the paths of knot k are cut into work items of P paths;
-// an item is run by one work-group.  Lanes take paths from an LDS counter and
-// refill as soon as their path stops (lane recycling, the on-device form of the
-// reference's 1000-step relaunch loop, Solvers/SolveLoop.cuh:59-74,
-// solveCUDA.cuh:118-168).  In persistent mode work-groups take items from a
-// device-wide atomic queue until it is empty.  Each path is stepped by the
-// control-variate Euler-Maruyama scheme (Integrators/FKACIntegrator.hpp:46-57)
-// and stopped by the Gobet-shifted test (Integrators/FKACIntegrator.hpp:168-186).
-// When an item's paths are done, the work-group reduces them in place: per
-// sub-batch of S paths the per-side RBF moments Phi_j = sum Y phi_j(X_tau)
-// (m exps per exit instead of the reference's m^2, Meshes/stencil.cpp:244-307),
-// then G_beta = -Psi^-T Phi per CSR entry, accumulated as sum and sum of squares
-// (batch-means variance) into per-item partials; a finalize kernel sums the
-// items of each knot in item order.  Every sum has a fixed order, so results do
-// not depend on scheduling.
+// Two kernels per assembly batch:
+//  * path_kernel: one path per lane.  In persistent form every lane whose path
+//    has stopped takes the next path of the batch from a device-wide counter
+//    (one atomic per wave per refill), so lanes idle only at the very end of the
+//    batch -- the on-device form of the reference's relaunch-and-refill loop
+//    (Solvers/SolveLoop.cuh:59-74, solveCUDA.cuh:118-168).  Each path is stepped
+//    by the control-variate Euler-Maruyama scheme (Integrators/FKACIntegrator.hpp:46-57),
+//    stopped by the Gobet-shifted test (Integrators/FKACIntegrator.hpp:168-186),
+//    and its exit record (projection, Y, score, side, steps) is written at its
+//    global path index.  The plain form launches one lane per path, no refill.
+//  * reduce_kernel: one work-group per work item of P paths of one knot.  Per
+//    sub-batch of S paths it forms the per-side RBF moments Phi_j = sum Y phi_j(X)
+//    (m exps per exit, where Meshes/stencil.cpp:244-307 spends m^2), then
+//    G_beta = -Psi^-T Phi per CSR column, and accumulates sums and sums of squares
+//    (batch-means variance) into per-item partials.  finalize_kernel adds the
+//    items of each knot in item order.  Every sum has a fixed order: results do
+//    not depend on scheduling.
 #pragma once
 #include "pdds_pde.h"
 #include "engine_types.h"
@@ -49,11 +51,10 @@ PDDS_HD int classify_kind(int kind, int gmask) {
     return v;
 }
 
-// Gaussian RBF exp(-|p - X|^2 / c^2), sqrt-then-square as BVPs/BVP.hpp:84-87
-PDDS_HD double rbf(double px, double py, double x, double y, double c) {
+// Gaussian RBF exp(-|p - X|^2 / c^2) (BVPs/BVP.hpp:84-87) with 1/c^2 precomputed
+PDDS_HD double rbf(double px, double py, double x, double y, double inv_c2) {
     double dx = px - x, dy = py - y;
-    double r = __builtin_sqrt(dx * dx + dy * dy);
-    return dexp(-(r * r) / (c * c));
+    return dexp(-(dx * dx + dy * dy) * inv_c2);
 }
 
 // ----------------------------------------------------------------------------
@@ -64,8 +65,7 @@ template <class R> struct PathState {
     R zc, xc;          // Kahan compensations (FP32 only)
     R pn1, pn2;        // pending normals of the odd step (FP32 only)
     int64_t st;
-    uint32_t path;
-    int plocal;
+    uint32_t path;     // path index within its knot
 };
 
 // One step of the control-variate EM scheme; coefficients at the old X are in co.
@@ -122,214 +122,129 @@ __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uin
     }
 }
 
+// Knot of global path g: the last knot whose first path is <= g.
+__device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int64_t g) {
+    int lo = 0, hi = n_knots;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (knots[mid].path_off <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ----------------------------------------------------------------------------
-// Assembly kernel
+// Path kernel
 // ----------------------------------------------------------------------------
 template <class R, int PDE, bool PERSIST>
-__global__ void __launch_bounds__(256) assemble_kernel(EngineArgs a) {
-    extern __shared__ double smem[];
-    __shared__ int64_t s_item;
-    __shared__ int s_next;
-    __shared__ unsigned long long s_steps;
-    __shared__ unsigned int s_cnt[8];
-    __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 4))) path_kernel(PathKernelArgs a) {
+    const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
+    const int lane = __lane_id();
+    PathState<R> s;
+    Coef<R> co;
+    R rect[4];
+    double rectd[4];
+    uint32_t kid = 0;
+    int gmask = 0;
+    int64_t g = -1;
+    bool active = false, done = false, fresh = false;
 
-    const int tid = threadIdx.x;
-    const int WG = blockDim.x;
-    const int64_t slot = blockIdx.x;
-    double *rx = a.rec_x + slot * a.rec_stride;
-    double *ry = a.rec_y + slot * a.rec_stride;
-    double *rY = a.rec_Y + slot * a.rec_stride;
-    double *rs = a.rec_score + slot * a.rec_stride;
-    int8_t *rside = a.rec_side + slot * a.rec_stride;
-    double *Phi = smem;
-    double *accS = smem + kNBG * a.ne_cap;
-    double *accQ = accS + a.nnz_cap;
+    auto start = [&](int64_t gg) {
+        const int k = find_knot(a.knots, a.n_knots, gg);
+        const KnotDesc &kd = a.knots[k];
+        const StencilDesc &sd = a.sten[kd.stencil];
+        for (int q = 0; q < 4; ++q) { rectd[q] = sd.rect[q]; rect[q] = (R)sd.rect[q]; }
+        gmask = sd.gmask;
+        kid = kd.knot_id;
+        g = gg;
+        s.path = (uint32_t)(gg - kd.path_off);
+        s.x = (R)kd.x0; s.y = (R)kd.y0;
+        s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
+        s.st = 0;
+        active = true;
+        fresh = true;
+    };
 
-    int64_t item = PERSIST ? -1 : a.item_base + (int64_t)blockIdx.x;
+    if (!PERSIST) {
+        const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (gg < a.total) start(gg);
+        else done = true;
+    }
+    // wave-private pool of consecutive paths [pool_next, pool_end), refilled kPoolChunk
+    // paths at a time from the device-wide counter (one atomic per kPoolChunk paths)
+    int64_t pool_next = 0, pool_end = 0;
+    bool drained = false;
     while (true) {
         if (PERSIST) {
-            if (tid == 0) s_item = (int64_t)atomicAdd((unsigned long long *)a.work_counter, 1ULL);
-            __syncthreads();
-            // the item is the same for the whole work-group: keep it (and every descriptor
-            // loaded through it) in scalar registers
-            const int64_t v = s_item;
-            item = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
-                             (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v));
-        }
-        if (item >= a.n_items) break;
-        const ItemDesc it = a.items[item];
-        const KnotDesc kd = a.knots[it.knot];
-        const StencilDesc &sd = a.sten[kd.stencil];
-        if (tid == 0) {
-            s_next = 0;
-            s_steps = 0ULL;
-            for (int q = 0; q < 8; ++q) s_cnt[q] = 0u;
-        }
-        __syncthreads();
-
-        // ---------------- path phase ----------------
-        {
-            const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
-            const R rect[4] = {(R)sd.rect[0], (R)sd.rect[1], (R)sd.rect[2], (R)sd.rect[3]};
-            PathState<R> s;
-            Coef<R> co;
-            bool active = false, exhausted = false;
-            unsigned long long my_steps = 0ULL;
-            while (true) {
-                bool fresh = false;
-                if (!active && !exhausted) {
-                    int p = atomicAdd(&s_next, 1);
-                    if (p < it.n) {
-                        s.plocal = p;
-                        s.path = (uint32_t)(it.p_begin + p);
-                        s.x = (R)kd.x0; s.y = (R)kd.y0;
-                        s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
-                        s.st = 0;
-                        active = true;
-                        fresh = true;
-                    } else {
-                        exhausted = true;
-                    }
+            const bool need = !active && !done;
+            const unsigned long long m = __ballot(need);
+            if (m) {
+                if (pool_next >= pool_end && !drained) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)kPoolChunk);
+                    base = __shfl(base, 0);
+                    pool_next = (int64_t)base;
+                    pool_end = pool_next + kPoolChunk < a.total ? pool_next + kPoolChunk : a.total;
+                    if (pool_next >= a.total) drained = true;
                 }
-                if (!__any(active)) break;
-                R d = 0, ex = 0, ey = 0, nx = 0, ny = 0;
-                int kind = 0;
-                if (active && !fresh) {
-                    R n1, n2;
-                    draw_normals<R>(s, a.seed, a.stream, kd.knot_id, n1, n2);
-                    em_step<R>(s, co, h, sqrth, n1, n2);
-                    s.st += 1;
-                    d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+                const int64_t avail = pool_end > pool_next ? pool_end - pool_next : 0;
+                const int rank = __popcll(m & ((1ULL << lane) - 1ULL));
+                if (need) {
+                    if (rank < avail) start(pool_next + rank);
+                    else if (drained) done = true;
                 }
-                if (active) pde_coef<PDE, R>(s.x, s.y, a.pp, co);
-                if (active && !fresh) {
-                    R ga = nx * co.s00 + ny * co.s10;
-                    R gb = nx * co.s01 + ny * co.s11;
-                    R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-                    bool go = d < beta * nrm * sqrth;
-                    bool capped = s.st >= a.max_steps;
-                    if (!go || capped) {
-                        int side;
-                        if constexpr (sizeof(R) == 8) side = classify_coord(sd.rect, a.glob, (double)ex, (double)ey);
-                        else side = classify_kind(kind, sd.gmask);
-                        double sc;
-                        if (side == 4) {
-                            R g = pde_g<PDE, R>(ex, ey, a.pp);
-                            sc = (double)(s.Z + s.Y * g + s.xi);
-                        } else {
-                            sc = (double)(s.Z + s.xi);
-                        }
-                        rx[s.plocal] = (double)ex;
-                        ry[s.plocal] = (double)ey;
-                        rY[s.plocal] = (double)s.Y;
-                        rs[s.plocal] = sc;
-                        rside[s.plocal] = (int8_t)side;
-                        atomicAdd(&s_cnt[side < 0 ? 5 : side], 1u);
-                        if (capped && go) atomicAdd(&s_cnt[6], 1u);
-                        my_steps += (unsigned long long)s.st;
-                        active = false;
-                    }
-                }
+                const int64_t cnt = __popcll(m);
+                pool_next += cnt < avail ? cnt : avail;
             }
-            atomicAdd(&s_steps, my_steps);
         }
-        __syncthreads();
-
-        // ---------------- reduction phase ----------------
-        const int ne = sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3];
-        const int nnz = sd.csr_nnz;
-        const int S = a.S;
-        const int nsb = (it.n + S - 1) / S;
-        int eoff[5];
-        eoff[0] = 0;
-        for (int q = 0; q < 4; ++q) eoff[q + 1] = eoff[q] + sd.msz[q];
-        for (int c = tid; c < nnz; c += WG) { accS[c] = 0.0; accQ[c] = 0.0; }
-        double Bs = 0.0, BBs = 0.0;
-        for (int g0 = 0; g0 < nsb; g0 += kNBG) {
-            const int nb = (nsb - g0) < kNBG ? (nsb - g0) : kNBG;
-            for (int e = tid; e < ne; e += WG) {
-                const int se = a.ent_side[sd.ent_off + e];
-                const double pxe = a.ent_x[sd.ent_off + e], pye = a.ent_y[sd.ent_off + e];
-                const double cs = sd.cshape[se];
-                for (int b = 0; b < nb; ++b) {
-                    const int p0 = (g0 + b) * S;
-                    const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
-                    double acc = 0.0;
-                    for (int p = p0; p < p1; ++p)
-                        if (rside[p] == se) acc = acc + rY[p] * rbf(pxe, pye, rx[p], ry[p], cs);
-                    Phi[b * a.ne_cap + e] = acc;
+        if (!__any(active)) break;
+        R d = 0, ex = 0, ey = 0, nx = 0, ny = 0;
+        int kind = 0;
+        if (active && !fresh) {
+            R n1, n2;
+            draw_normals<R>(s, a.seed, a.stream, kid, n1, n2);
+            em_step<R>(s, co, h, sqrth, n1, n2);
+            s.st += 1;
+            d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        }
+        if (active) pde_coef<PDE, R>(s.x, s.y, a.pp, co);
+        if (active && !fresh) {
+            R ga = nx * co.s00 + ny * co.s10;
+            R gb = nx * co.s01 + ny * co.s11;
+            R nrm = RM<R>::sqrt(ga * ga + gb * gb);
+            const bool go = d < beta * nrm * sqrth;
+            const bool capped = s.st >= a.max_steps;
+            if (!go || capped) {
+                int side;
+                if constexpr (sizeof(R) == 8) side = classify_coord(rectd, a.glob, (double)ex, (double)ey);
+                else side = classify_kind(kind, gmask);
+                double sc;
+                if (side == 4) {
+                    R gv = pde_g<PDE, R>(ex, ey, a.pp);
+                    sc = (double)(s.Z + s.Y * gv + s.xi);
+                } else {
+                    sc = (double)(s.Z + s.xi);
                 }
+                a.rec.x[g] = (double)ex;
+                a.rec.y[g] = (double)ey;
+                a.rec.Y[g] = (double)s.Y;
+                a.rec.score[g] = sc;
+                a.rec.side[g] = (int8_t)side;
+                a.rec.steps[g] = (int32_t)s.st | ((capped && go) ? kCappedBit : 0);
+                active = false;
             }
-            for (int b = tid; b < nb; b += WG) {
-                const int p0 = (g0 + b) * S;
-                const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
-                double bs = 0.0, bb = 0.0;
-                for (int p = p0; p < p1; ++p) { double v = rs[p]; bs = bs + v; bb = bb + v * v; }
-                s_bsb[b] = bs;
-                s_bbsb[b] = bb;
-            }
-            __syncthreads();
-            for (int c = tid; c < nnz; c += WG) {
-                double G[kNBG];
-#pragma unroll
-                for (int b = 0; b < kNBG; ++b) G[b] = 0.0;
-                const int ec[2] = {a.csr_e1[sd.csr_off + c], a.csr_e2[sd.csr_off + c]};
-                for (int q = 0; q < 2; ++q) {
-                    const int e = ec[q];
-                    if (e < 0) continue;
-                    const int se = a.ent_side[sd.ent_off + e];
-                    const int m = sd.msz[se];
-                    const int el = e - eoff[se];
-                    const double *A = a.ipsi + sd.ipsi_off[se];
-                    double H[kNBG];
-#pragma unroll
-                    for (int b = 0; b < kNBG; ++b) H[b] = 0.0;
-                    for (int j = 0; j < m; ++j) {
-                        const double aj = A[(int64_t)j * m + el];
-                        const double *ph = Phi + eoff[se] + j;
-#pragma unroll
-                        for (int b = 0; b < kNBG; ++b)
-                            if (b < nb) H[b] = H[b] + aj * ph[b * a.ne_cap];
-                    }
-#pragma unroll
-                    for (int b = 0; b < kNBG; ++b) G[b] = G[b] + (-H[b]);
-                }
-                double sum = accS[c], q2 = accQ[c];
-                for (int b = 0; b < nb; ++b) {
-                    const int p0 = (g0 + b) * S;
-                    const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
-                    sum = sum + G[b];
-                    q2 = q2 + (G[b] * G[b]) / (double)(p1 - p0);
-                }
-                accS[c] = sum;
-                accQ[c] = q2;
-            }
-            if (tid == 0)
-                for (int b = 0; b < nb; ++b) { Bs = Bs + s_bsb[b]; BBs = BBs + s_bbsb[b]; }
-            __syncthreads();
         }
-        for (int c = tid; c < nnz; c += WG) {
-            a.partials[it.part_off + c] = accS[c];
-            a.partials[it.part_off + nnz + c] = accQ[c];
-        }
-        if (tid == 0) {
-            ItemStat st;
-            st.Bs = Bs;
-            st.BBs = BBs;
-            st.steps = s_steps;
-            for (int q = 0; q < 6; ++q) st.cnt[q] = s_cnt[q];
-            st.capped = s_cnt[6];
-            st.pad = 0;
-            a.stats[item] = st;
-        }
-        __syncthreads();
-        if (!PERSIST) break;
+        fresh = false;
     }
 }
 
-// One block per knot: sum the knot's items in item order (defined in engine_f64.hip).
-__global__ void finalize_kernel(FinalizeArgs f);
+// ----------------------------------------------------------------------------
+// Reduction kernel: one work-group per work item
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a);
+
+// One block per knot: sum the knot's items in item order.
+__global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);
 
 // ----------------------------------------------------------------------------
 // Per-path kernel (MCinCUDA shape): one path per lane, records written out.
@@ -377,8 +292,5 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
         if (os) os[i] = s.st - st0;
     }
 }
-
-// Device LUT evaluation for tests (FP64 arithmetic, field f)
-__global__ void lut_eval_kernel(LutView L, int field, int64_t n, const double *x, const double *y, double *out);
 
 }  // namespace pdds

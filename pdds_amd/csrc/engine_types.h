@@ -5,14 +5,17 @@
 
 namespace pdds {
 
-constexpr int kNBG = 8;  // sub-batches reduced per pass over Psi^-1
+constexpr int kNBG = 8;                    // sub-batches reduced per pass over Psi^-1
+constexpr int32_t kCappedBit = 1 << 30;    // record flag: path stopped by the step cap
+constexpr int64_t kPoolChunk = 256;        // paths a wave takes from the device-wide queue at once
 
-// One Monte Carlo knot of an assembly launch.
+// One Monte Carlo knot of an assembly batch.
 struct KnotDesc {
     double x0, y0;          // start point
-    uint32_t knot_id;       // Philox counter word 2 (global knot index)
+    uint32_t knot_id;       // Philox counter word 2 (global knot index / request point id)
     int32_t stencil;        // index into the stencil table
     int64_t n_paths;
+    int64_t path_off;       // global index of the knot's first path in the batch (ascending)
     int64_t out_off;        // first value of the knot's G row in the CSR value array
     int64_t item_begin;     // first work item of the knot
     int64_t item_count;
@@ -22,7 +25,7 @@ struct KnotDesc {
 // and for each CSR column of the row up to two contributing side entries.
 struct StencilDesc {
     double rect[4];
-    double cshape[4];       // Gaussian shape c per side
+    double inv_c2[4];       // 1 / c^2 of the Gaussian shape c per side
     int64_t ipsi_off[4];    // row-major m x m blocks in the ipsi array
     int32_t msz[4];
     int32_t ent_off;        // first side entry in ent_x / ent_y / ent_side
@@ -34,7 +37,7 @@ struct StencilDesc {
 struct ItemDesc {
     int32_t knot;           // index into the knot table
     int32_t n;              // paths in the item
-    int64_t p_begin;        // first path index (Philox counter word 1)
+    int64_t p_begin;        // first path index within the knot
     int64_t part_off;       // 2*nnz partials (sum, sum of squares / n_beta)
 };
 
@@ -53,7 +56,29 @@ struct KnotStat {
     unsigned long long cnt[6];
 };
 
-struct EngineArgs {
+// Per-path exit records, indexed by global path index.
+struct Records {
+    double *x, *y, *Y, *score;
+    int8_t *side;
+    int32_t *steps;
+};
+
+struct PathKernelArgs {
+    const KnotDesc *knots;
+    int32_t n_knots;
+    const StencilDesc *sten;
+    int64_t total;          // paths in the batch
+    int64_t base;           // plain launches: first path of this launch
+    unsigned long long *counter;
+    Records rec;
+    double glob[4];
+    double h, sqrth, beta;
+    uint32_t seed, stream;
+    int64_t max_steps;
+    PdeParams pp;
+};
+
+struct ReduceArgs {
     const KnotDesc *knots;
     const StencilDesc *sten;
     const double *ent_x, *ent_y;
@@ -61,21 +86,12 @@ struct EngineArgs {
     const int32_t *csr_e1, *csr_e2;
     const double *ipsi;
     const ItemDesc *items;
-    int64_t n_items;
-    int64_t item_base;      // non-persistent launches: item = item_base + blockIdx.x
+    int64_t item_base;
+    Records rec;
     ItemStat *stats;
     double *partials;
-    double *rec_x, *rec_y, *rec_Y, *rec_score;
-    int8_t *rec_side;
-    int64_t rec_stride;
-    unsigned long long *work_counter;
     int32_t ne_cap, nnz_cap;
-    double glob[4];
-    double h, sqrth, beta;
-    uint32_t seed, stream;
-    int64_t max_steps;
     int32_t S;
-    PdeParams pp;
 };
 
 struct FinalizeArgs {
@@ -105,9 +121,10 @@ struct PathsArgs {
     int64_t *steps2;
 };
 
-// Launchers (defined in engine_f64.hip / engine_f32.hip / engine_common.hip)
-hipError_t launch_assemble_f64(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a);
-hipError_t launch_assemble_f32(int pde, bool persistent, int grid, int wg, size_t shmem, hipStream_t s, const EngineArgs &a);
+// Launchers (defined in engine_f64.hip / engine_f32.hip)
+hipError_t launch_path_f64(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a);
 hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);
 hipError_t launch_paths_f32(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);
 hipError_t launch_finalize(int grid, int wg, hipStream_t s, const FinalizeArgs &f);

@@ -306,57 +306,70 @@ PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kin
 }
 
 // --------------------------------------------------------------- LUT -----
-// Cubic B-spline coefficient table [ny][nx][nf] (FP32), t = (x - ox) * sx.
+// Cubic B-spline coefficient table [ny][nx][stride] (FP32, stride 4 or 8 floats per texel,
+// 16-byte aligned), t = (x - ox) * sx.  Like the reference's FP32 texture lookups
+// (Integrators/cubicTex2D.cu:64-87), the 16-tap weighted sum runs in FP32 -- here with
+// explicit fma -- whatever the path precision; only the coordinate is formed in R.
 struct LutView {
     const float *coef;
-    int nx, ny, nf;
+    int nx, ny, stride, nf;
     double ox, oy, sx, sy;
 };
 
-template <class R> PDDS_HD void bspline_w(R f, R w[4]) {
-    R f2 = f * f, f3 = f2 * f, om = (R)1 - f;
-    const R s6 = (R)(1.0 / 6.0);
+// Uniform cubic B-spline weights of taps -1..2 at fraction f (Horner forms, fma)
+PDDS_HD void bspline_wf(float f, float w[4]) {
+    const float s6 = 1.0f / 6.0f;
+    const float f2 = f * f, om = 1.0f - f;
     w[0] = (om * om * om) * s6;
-    w[1] = ((R)3 * f3 - (R)6 * f2 + (R)4) * s6;
-    w[2] = ((R)-3 * f3 + (R)3 * f2 + (R)3 * f + (R)1) * s6;
-    w[3] = f3 * s6;
+    w[1] = __builtin_fmaf(__builtin_fmaf(3.0f, f, -6.0f), f2, 4.0f) * s6;
+    w[2] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-3.0f, f, 3.0f), f, 3.0f), f, 1.0f) * s6;
+    w[3] = (f2 * f) * s6;
 }
 
-// Evaluate fields [0, NF) at (x,y): sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
+// Evaluate fields [0, NF) (NF <= stride) at (x,y):
+//   sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
 template <class R, int NF>
 PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
+    static_assert(NF <= 8, "at most 8 fields per texel");
     R t = (x - (R)L.ox) * (R)L.sx;
     R v = (y - (R)L.oy) * (R)L.sy;
     t = RM<R>::fmin(RM<R>::fmax(t, (R)-2), (R)L.nx + (R)1);
     v = RM<R>::fmin(RM<R>::fmax(v, (R)-2), (R)L.ny + (R)1);
-    R fi = RM<R>::floor(t), fj = RM<R>::floor(v);
-    R wx[4], wy[4];
-    bspline_w<R>(t - fi, wx);
-    bspline_w<R>(v - fj, wy);
-    int i = (int)fi, j = (int)fj;
-    R acc[NF];
+    const R fi = RM<R>::floor(t), fj = RM<R>::floor(v);
+    float wx[4], wy[4];
+    bspline_wf((float)(t - fi), wx);
+    bspline_wf((float)(v - fj), wy);
+    const int i = (int)fi, j = (int)fj;
+    float acc[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) acc[f] = (R)0;
+    for (int f = 0; f < NF; ++f) acc[f] = 0.0f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         int jj = j - 1 + b;
         jj = jj < 0 ? 0 : (jj > L.ny - 1 ? L.ny - 1 : jj);
-        R row[NF];
+        float row[NF];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) row[f] = (R)0;
+        for (int f = 0; f < NF; ++f) row[f] = 0.0f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             int ii = i - 1 + a;
             ii = ii < 0 ? 0 : (ii > L.nx - 1 ? L.nx - 1 : ii);
-            const float *cp = L.coef + ((int64_t)jj * L.nx + ii) * L.nf;
+            const float4 *cp = (const float4 *)(L.coef + ((int64_t)jj * L.nx + ii) * L.stride);
+            float c[8];
+            const float4 q0 = cp[0];
+            c[0] = q0.x; c[1] = q0.y; c[2] = q0.z; c[3] = q0.w;
+            if constexpr (NF > 4) {
+                const float4 q1 = cp[1];
+                c[4] = q1.x; c[5] = q1.y; c[6] = q1.z; c[7] = q1.w;
+            }
 #pragma unroll
-            for (int f = 0; f < NF; ++f) row[f] = row[f] + wx[a] * (R)cp[f];
+            for (int f = 0; f < NF; ++f) row[f] = __builtin_fmaf(wx[a], c[f], row[f]);
         }
 #pragma unroll
-        for (int f = 0; f < NF; ++f) acc[f] = acc[f] + wy[b] * row[f];
+        for (int f = 0; f < NF; ++f) acc[f] = __builtin_fmaf(wy[b], row[f], acc[f]);
     }
 #pragma unroll
-    for (int f = 0; f < NF; ++f) out[f] = acc[f];
+    for (int f = 0; f < NF; ++f) out[f] = (R)acc[f];
 }
 
 }  // namespace pdds

@@ -58,6 +58,33 @@ def test_fp64_paths_bit_exact(eng, case):
     assert got["counters"]["path_steps"] == int(ref["steps"].sum())
 
 
+def _lut_pair(eng, n=96):
+    """The same 5-field coefficient table on the device and in the oracle."""
+    from pdds_amd.workloads import varcoef_fields
+    xs = np.linspace(-1, 1, n)
+    X, Y = np.meshgrid(xs, xs)
+    samples = varcoef_fields(X, Y)
+    lut = eng.lut(samples, (-1, -1, 1, 1), mode=0)
+    d = 2.0 / (n - 1)
+    olut = {"coef": O.lut_prefilter(samples), "NX": n + 2, "NY": n + 2, "nf": 5, "ox": -1 - d, "oy": -1 - d,
+            "sx": 1 / d, "sy": 1 / d}
+    return lut, olut
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_fp64_lut_paths_bit_exact(eng, precision):
+    from pdds_amd.engine import bvp, default_run
+    lut, olut = _lut_pair(eng)
+    m = _product_mesh((-1.0, -1.0), (1.0, 1.0), 4, 8, FIX_FAC)
+    k = 30
+    st = m.stencil(k)
+    got = eng.paths(bvp("varcoef_lut", lut=lut), default_run(h=1e-3, seed=9), m.x0[k], m.y0[k], st["rect"], k, 0, 2000)
+    ref = O.simulate(O.problem(O.PDE_VARCOEF_LUT, lut=olut), O.run_params(1e-3, 9, threads=8), m.x0[k], m.y0[k],
+                     st["rect"], k, 0, 2000)
+    for key in ("steps", "xt", "yt", "Y", "Z", "xi"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
 def test_fp64_phase2_bit_exact(eng):
     from pdds_amd.engine import bvp, default_run
     m = _product_mesh()

@@ -1,0 +1,121 @@
+"""The oracle's geometry and Monte Carlo pieces against closed forms.
+
+Rectangle2D (BVPs/Domains/rectangle.hpp:7-229) has exact answers for points in each
+region.  The Feynman-Kac estimator with the Gobet shift converges to the manufactured
+u at weak order h: a dropped term (f, c*Y, the control variate), a wrong sign or a
+wrong index shows up as an O(1) error or as a bias that does not shrink with h.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+R = [-1.0, -2.0, 3.0, 2.0]   # centre (1, 0), half sides (2, 2)
+
+
+@pytest.mark.parametrize("x,y,d,ex,ey,nx,ny", [
+    (2.5, 0.5, -0.5, 3.0, 0.5, 1, 0),      # inside, closest to the east side
+    (1.2, 1.9, -0.1, 1.2, 2.0, 0, 1),      # inside, north
+    (-0.8, -0.5, -0.2, -1.0, -0.5, -1, 0), # inside, west
+    (0.5, -1.7, -0.3, 0.5, -2.0, 0, -1),   # inside, south
+    (3.5, 1.0, 0.5, 3.0, 1.0, 1, 0),       # outside east
+    (0.0, 2.25, 0.25, 0.0, 2.0, 0, 1),     # outside north
+])
+def test_rect_dist_sides(x, y, d, ex, ey, nx, ny):
+    got = O.rect_dist(R, x, y)
+    assert got[0] == pytest.approx(d, abs=1e-15)
+    assert got[1:] == pytest.approx((ex, ey, nx, ny), abs=1e-15)
+
+
+def test_rect_dist_corner_and_tie():
+    d, ex, ey, nx, ny = O.rect_dist(R, 4.0, 3.0)
+    assert d == pytest.approx(np.sqrt(2.0)) and (ex, ey) == (3.0, 2.0)
+    assert nx == ny == pytest.approx(np.float32(0.70710678118))
+    # tie (equal distance to the north and east sides): the y side wins ("dx-hx > dy-hy" is false)
+    d, ex, ey, nx, ny = O.rect_dist(R, 2.5, 1.5)
+    assert (ex, ey, nx, ny) == (2.5, 2.0, 0.0, 1.0) and d == pytest.approx(-0.5)
+    # the centre itself falls in the third quadrant branch (x <= cx, y <= cy)
+    d, ex, ey, nx, ny = O.rect_dist(R, 1.0, 0.0)
+    assert (ey, ny) == (-2.0, -1.0)
+
+
+def point_estimate(pde, x0, y0, h, n, seed=3, k2=0.0):
+    pb = O.problem(pde, k2=k2)
+    run = O.run_params(h, seed, threads=8)
+    glob = np.array([-1.0, -1.0, 1.0, 1.0])
+    p = O.simulate(pb, run, x0, y0, glob, 0, 0, n)
+    g = np.array([O.pde_eval(pb, a, b)["g"] for a, b in zip(p["xt"], p["yt"])])
+    score = p["Z"] + p["Y"] * g + p["xi"]
+    u = O.pde_eval(pb, x0, y0)["u"]
+    return score.mean(), score.std() / np.sqrt(n), u, p
+
+
+@pytest.mark.parametrize("pde,k2", [(O.PDE_SINSIN, 0.0), (O.PDE_HELMHOLTZ, 4.0), (O.PDE_VARCOEF, 0.0)])
+def test_feynman_kac_point_estimate_weak_order_one(pde, k2):
+    """The estimate approaches u with bias c*h: at h = 1.6e-2 and 4e-3 (bias far above the
+    standard error) the error ratio is near 4, and at h = 1e-3 the error is below 12 h."""
+    x0, y0 = 0.3, -0.2
+    errs = {}
+    for h in (1.6e-2, 4e-3, 1e-3):
+        est, se, u, _ = point_estimate(pde, x0, y0, h, 20000, k2=k2)
+        errs[h] = (est - u, se)
+        assert abs(est - u) < 4 * se + 12.0 * h, (pde, h, est, u, se)
+    (e1, s1), (e2, s2) = errs[1.6e-2], errs[4e-3]
+    if abs(e1) > 10 * s1:
+        assert 2.5 < e1 / e2 < 6.5, (e1, e2)
+
+
+def test_control_variate_is_mean_zero_and_reduces_variance():
+    pb = O.problem(O.PDE_SINSIN)
+    run = O.run_params(1e-3, 5, threads=8)
+    glob = np.array([-1.0, -1.0, 1.0, 1.0])
+    p = O.simulate(pb, run, 0.1, 0.4, glob, 0, 0, 6000)
+    g = np.array([O.pde_eval(pb, a, b)["g"] for a, b in zip(p["xt"], p["yt"])])
+    plain = p["Z"] + p["Y"] * g
+    assert abs(p["xi"].mean()) < 4 * p["xi"].std() / np.sqrt(len(g))
+    assert (plain + p["xi"]).var() < 0.05 * plain.var()
+
+
+def torsion_centre():
+    """w(0,0) for -Lap w = 1 on [-1,1]^2, w = 0 on the boundary: with
+    (1-x^2)/2 = sum_{k odd} a_k cos(k pi x/2), a_k = 16 (-1)^((k-1)/2) / (k pi)^3,
+    w = (1-x^2)/2 - sum a_k cos(k pi x/2) cosh(k pi y/2) / cosh(k pi/2)."""
+    s = 0.0
+    for k in range(1, 400, 2):
+        s += ((-1) ** ((k - 1) // 2)) * 16.0 / (np.pi ** 3 * k ** 3) / np.cosh(k * np.pi / 2)
+    return 0.5 - s
+
+
+def test_torsion_series():
+    # sum_{n>=0} (-1)^n/(2n+1)^3 = pi^3/32 makes the 1-D part exactly 1/2 at x = 0
+    assert sum(16.0 * (-1) ** ((k - 1) // 2) / (np.pi * k) ** 3 for k in range(1, 20001, 2)) == pytest.approx(0.5)
+    assert torsion_centre() == pytest.approx(0.2946854, abs=1e-6)
+
+
+def test_exit_time_of_brownian_motion_in_a_square():
+    """sigma = sqrt2 I (generator: Laplacian), paths from the centre of [-1,1]^2:
+    E[tau] = torsion function at the centre."""
+    w00 = torsion_centre()
+    pb = O.problem(O.PDE_SINSIN)
+    for h, tol in ((1e-3, 0.02),):
+        run = O.run_params(h, 11, beta=0.0, threads=8)
+        p = O.simulate(pb, run, 0.0, 0.0, np.array([-1.0, -1.0, 1.0, 1.0]), 0, 0, 20000)
+        tau = p["steps"] * h
+        se = tau.std() / np.sqrt(len(tau))
+        # beta = 0: exit detected on the discrete grid, overshoot bias ~ 0.58 sqrt(2h) in distance
+        assert abs(tau.mean() - w00) < 4 * se + tol, (tau.mean(), w00)
+
+
+def test_gobet_shift_removes_the_discrete_monitoring_bias():
+    """Without the shift (beta = 0) the discretely monitored exit is late by O(sqrt(h));
+    with beta = -0.5826 the mean exit time moves onto the continuous one."""
+    w00 = torsion_centre()
+    pb = O.problem(O.PDE_SINSIN)
+    h = 4e-3
+    out = {}
+    for beta in (0.0, -0.5826):
+        run = O.run_params(h, 17, beta=beta, threads=8)
+        p = O.simulate(pb, run, 0.0, 0.0, np.array([-1.0, -1.0, 1.0, 1.0]), 0, 0, 40000)
+        out[beta] = (p["steps"] * h).mean()
+    assert out[0.0] - w00 > 0.01           # late exit without the shift
+    assert abs(out[-0.5826] - w00) < 0.006  # shifted boundary corrects it

@@ -20,7 +20,7 @@
  *                           per-path exit records, phase 1 and optional phase 2
  *   pdds_solve_knot         EMFKACSolver::Solve_OMP_Analytic with a stencil (Solvers/EMFKACSolver.hpp:711-729):
  *                           one knot's G row, G variances, B and B second moment
- *   pdds_point_eval         EMFKACSolver::Solve_OMP_Analytic without a stencil (Solvers/EMFKACSolver.hpp:687-692):
+ *   pdds_point_eval         EMFKACSolver::Solve_OMP_Analytic without a stencil (Solvers/EMFKACSolver.hpp:686-691):
  *                           Feynman-Kac estimate of u at one point
  *   pdds_assemble           the worker loop of PDDSparseGM::Solve (PDDAlgorithms/PDDSparseGM.cpp:795-848):
  *                           every knot of a mesh in one persistent launch, straight into CSR

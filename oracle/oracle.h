@@ -6,7 +6,7 @@
  * (as oracle/liboracle.so, through oracle/oracle.py); the product library
  * never links it.  Each function cites the reference file:line it restates.
  *
- * Parity status (also stated in DESIGN.md):
+ * Parity status:
  *   orc_philox4x32_10          pinned: published Philox4x32-10 known answers
  *   orc_exp/log/sinpi/...      pinned: libm within a few ulp, mpmath spot checks
  *   orc_rect_dist              pinned: closed-form geometry cases

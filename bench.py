@@ -32,11 +32,12 @@ PEAK_FP64_TFLOPS = 78.6
 
 
 def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
-    """The FP64 C/OpenMP oracle (a restatement of the reference CPU path) on a bounded sample:
-    a seeded random subset of knots, paths per knot scaled to about budget_s seconds."""
+    """The FP64 C/OpenMP oracle (a restatement of the reference CPU path) on a bounded sample of
+    the same workload: chunks of paths on seeded random interior knots until about budget_s
+    seconds of stepping have elapsed."""
     from oracle import oracle as O
     from pdds_amd.workloads import varcoef_fields
-    threads = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     if prob_name == "varcoef_lut":
         n = 512
         xs = np.linspace(-1, 1, n)
@@ -49,28 +50,43 @@ def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
         pb = O.problem({"monegros": O.PDE_MONEGROS, "sinsin": O.PDE_SINSIN}[prob_name])
     rng = np.random.default_rng(1234)
     interior = np.flatnonzero(~mesh.on_boundary)
-    knots = rng.choice(interior, size=min(8, len(interior)), replace=False)
+    order = rng.permutation(interior)
     run = O.run_params(h, seed, threads=threads)
-    # calibrate on a small batch, then size the sample
-    k0 = int(knots[0])
-    st0 = mesh.stencil(k0)
-    t = time.perf_counter()
-    cal = O.simulate(pb, run, mesh.x0[k0], mesh.y0[k0], st0["rect"], k0, 0, 256)
-    rate = cal["steps"].sum() / max(time.perf_counter() - t, 1e-6)
-    steps_per_path = max(cal["steps"].mean(), 1.0)
-    per_knot = int(max(256, min(n_paths_ref, budget_s * rate / (len(knots) * steps_per_path))))
+    # time-bounded sample: chunks of paths on seeded random knots until budget_s has elapsed
+    chunk = 65536
     steps = 0
+    used = []
     t = time.perf_counter()
-    for k in knots:
-        k = int(k)
+    while time.perf_counter() - t < budget_s:
+        k = int(order[len(used) % len(order)])
         st = mesh.stencil(k)
-        paths = O.simulate(pb, run, mesh.x0[k], mesh.y0[k], st["rect"], k, 0, per_knot)
+        paths = O.simulate(pb, run, mesh.x0[k], mesh.y0[k], st["rect"], k, 0, chunk)
         steps += int(paths["steps"].sum())
+        used.append(k)
     t_sim = time.perf_counter() - t
     return {"value": steps / t_sim, "unit": "path-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d knots x %d paths (seeded subset of %d interior knots), phase-1 stepping, "
-                      "FP64 C/OpenMP oracle, %.1f s" % (len(knots), per_knot, len(interior), t_sim),
+            "sample": "%d knots x %d paths (seeded random interior knots of %d), %d path-steps, phase-1 "
+                      "stepping, FP64 C/OpenMP oracle, %.1f s" % (len(used), chunk, len(interior), steps, t_sim),
             "cpu": platform.processor() or platform.machine()}
+
+
+def copy_calibration(n=1 << 30, reps=3):
+    """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it gives the factors that
+    turn those counters into HBM bytes for the path kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(n)):
+        raise RuntimeError("hipMemset failed")
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3):
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n, "bytes_written_per_copy": n, "copies": reps}}))
 
 
 def rms_error(eng, mesh, prob, out):
@@ -89,7 +105,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_cfg2.json"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device-to-device copies of 1 GiB (the reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the path engine has no multi-GPU form in this build "
@@ -158,7 +179,7 @@ def main():
         "device": info,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "assemble_kernel<double, PDE_VARCOEF_LUT, persistent>",
+                     "kernel": "path_kernel<double, PDE_VARCOEF_LUT, persistent>",
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
                      "note": "FP64 VALU-bound (no dense contraction); peak = MI355X FP64 vector = FP64 matrix "
                              "78.6 TFLOP/s; achieved counts %g algorithmic flops per path-step"

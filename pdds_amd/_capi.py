@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpdds.so")
+LIB_PATH = os.environ.get("PDDS_LIB") or os.path.join(_HERE, "libpdds.so")
 
 OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_SINGULAR, ERR_NOT_READY, ERR_FULL = range(8)
 PDE_MONEGROS, PDE_SINSIN, PDE_VARCOEF, PDE_VARCOEF_LUT, PDE_HELMHOLTZ = range(5)

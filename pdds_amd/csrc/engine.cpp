@@ -275,8 +275,9 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             off += k.n;
         }
         items_all += (int64_t)items.size();
-        const size_t shmem = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap) * sizeof(double);
-        if (shmem > 160 * 1024) return fail(e, PDDS_ERR_UNSUPPORTED, "stencil too large for the LDS reduction buffers");
+        const size_t shmem = reduce_lds_bytes(ne_cap, nnz_cap, S);
+        if (shmem > 160 * 1024 || S > 65535)
+            return fail(e, PDDS_ERR_UNSUPPORTED, "stencil or sub-batch too large for the LDS reduction buffers");
         KnotDesc *dk; ItemDesc *dit;
         if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
         if ((st = upload(e, B_ITEMS, items, &dit))) return st;
@@ -291,8 +292,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st; dstats = (ItemStat *)p;
         if ((st = dbuf(e, B_PART, (size_t)std::max<int64_t>(part, 1) * sizeof(double), &p))) return st; dpart = (double *)p;
         if ((st = dbuf(e, B_KSTAT, (size_t)nk * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
-        if ((st = dbuf(e, B_COUNTER, 64, &p))) return st; dctr = (unsigned long long *)p;
-        HIPCHK(hipMemsetAsync(dctr, 0, 64, e->stream));
+        if ((st = dbuf(e, B_COUNTER, kMaxQueues * sizeof(unsigned long long), &p))) return st; dctr = (unsigned long long *)p;
+        HIPCHK(hipMemsetAsync(dctr, 0, kMaxQueues * sizeof(unsigned long long), e->stream));
         // path kernel
         PathKernelArgs pa;
         std::memset(&pa, 0, sizeof(pa));
@@ -307,6 +308,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         } else {
             grid = (int)std::min<int64_t>((tot + wg - 1) / wg, (int64_t)1 << 30);
         }
+        pa.n_queues = persistent ? std::min(kMaxQueues, grid) : 1;
         grid_used = std::max(grid_used, grid);
         HIPCHK(hipEventRecord(e->ev[0], e->stream));
         const int64_t per_launch = persistent ? tot : (int64_t)grid * wg;
@@ -532,24 +534,44 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
         v.sx = (double)nx / (x1 - x0); v.sy = (double)ny / (y1 - y0);
         v.ox = x0 + 0.5 / v.sx; v.oy = y0 + 0.5 / v.sy;
     }
-    // texels padded to 4 or 8 floats so that a tap is one or two 16-byte vector loads
-    const int stride = nf <= 4 ? 4 : 8;
-    std::vector<float> padded((size_t)NX * NY * stride, 0.0f);
-    for (size_t t = 0; t < (size_t)NX * NY; ++t)
-        for (int f = 0; f < nf; ++f) padded[t * stride + f] = coef[t * nf + f];
-    v.nx = NX; v.ny = NY; v.nf = nf; v.stride = stride;
+    // device layout (see LutView): replicated border of kLutPad texels below and
+    // kLutPad + 1 above in both directions; fields 0..3 in a float4 plane; field 4 as
+    // x-quads (nf == 5) or fields 4..7 per texel (nf > 5) in a second float4 plane
+    const int PX = NX + 2 * kLutPad + 1, PY = NY + 2 * kLutPad + 1;
+    const size_t nt = (size_t)PX * PY;
+    const bool has_b = nf > 4;
+    if (nt * 16 >= ((size_t)1 << 32)) return fail(e, PDDS_ERR_UNSUPPORTED, "LUT larger than 4 GiB per plane");
+    std::vector<float> pa(nt * 4, 0.0f), pb(has_b ? nt * 4 : 4, 0.0f);
+    auto src = [&](int pj, int pi, int f) {
+        const int j = std::min(std::max(pj - kLutPad, 0), NY - 1);
+        const int i = std::min(std::max(pi - kLutPad, 0), NX - 1);
+        return coef[((size_t)j * NX + i) * nf + f];
+    };
+    for (int pj = 0; pj < PY; ++pj)
+        for (int pi = 0; pi < PX; ++pi) {
+            const size_t t = (size_t)pj * PX + pi;
+            for (int f = 0; f < std::min(nf, 4); ++f) pa[t * 4 + f] = src(pj, pi, f);
+            if (nf == 5)
+                for (int a = 0; a < 4; ++a) pb[t * 4 + a] = src(pj, std::min(pi + a, PX - 1), 4);
+            else
+                for (int f = 4; f < nf; ++f) pb[t * 4 + (f - 4)] = src(pj, pi, f);
+        }
+    v.nx = NX; v.ny = NY; v.px = PX; v.nf = nf;
     pdds_lut *l = new pdds_lut();
     l->e = e;
-    if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, padded.size() * sizeof(float)) != hipSuccess) {
+    const size_t bytes_a = pa.size() * sizeof(float), bytes_b = has_b ? pb.size() * sizeof(float) : 0;
+    if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, bytes_a + bytes_b) != hipSuccess) {
         delete l;
         return fail(e, PDDS_ERR_NOMEM, "LUT allocation failed");
     }
-    if (hipMemcpy(l->d, padded.data(), padded.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(l->d, pa.data(), bytes_a, hipMemcpyHostToDevice) != hipSuccess ||
+        (bytes_b && hipMemcpy((char *)l->d + bytes_a, pb.data(), bytes_b, hipMemcpyHostToDevice) != hipSuccess)) {
         (void)hipFree(l->d);
         delete l;
         return fail(e, PDDS_ERR_DEVICE, "LUT upload failed");
     }
-    v.coef = l->d;
+    v.a = (const float4 *)l->d;
+    v.b = bytes_b ? (const float4 *)((const char *)l->d + bytes_a) : nullptr;
     l->view = v;
     l->nx = nx; l->ny = ny; l->nf = nf; l->mode = mode;
     *out = l;
@@ -872,12 +894,12 @@ extern "C" pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_
     for (int64_t k = 0; k < nnz; ++k)
         if (cols[k] < 0 || cols[k] >= n) return fail(e, PDDS_ERR_INVALID, "column out of range");
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
-    void *drp, *dci, *dv, *db, *dx, *dw, *dit;
+    void *drp, *dci, *dv, *db, *dx, *dw;
     pdds_status st;
+    const int mi = maxit > 0 ? maxit : 2 * n;
     if ((st = dbuf(e, B_P0, (size_t)(n + 1) * 8, &drp)) || (st = dbuf(e, B_P1, (size_t)std::max<int64_t>(nnz, 1) * 4, &dci)) ||
         (st = dbuf(e, B_P2, (size_t)std::max<int64_t>(nnz, 1) * 8, &dv)) || (st = dbuf(e, B_P3, (size_t)n * 8, &db)) ||
-        (st = dbuf(e, B_P4, (size_t)n * 8, &dx)) || (st = dbuf(e, B_P5, (size_t)n * 6 * 8, &dw)) ||
-        (st = dbuf(e, B_P6, 64, &dit)))
+        (st = dbuf(e, B_P4, (size_t)n * 8, &dx)) || (st = dbuf(e, B_P5, bicgstab_work_bytes(n, mi), &dw)))
         return st;
     HIPCHK(hipMemcpyAsync(drp, row_ptr, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, e->stream));
     if (nnz > 0) {
@@ -885,12 +907,10 @@ extern "C" pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_
         HIPCHK(hipMemcpyAsync(dv, G, (size_t)nnz * 8, hipMemcpyHostToDevice, e->stream));
     }
     HIPCHK(hipMemcpyAsync(db, B, (size_t)n * 8, hipMemcpyHostToDevice, e->stream));
-    const int mi = maxit > 0 ? maxit : 2 * n;
-    HIPCHK(bicgstab_device(n, (const int64_t *)drp, (const int32_t *)dci, (const double *)dv, (const double *)db,
-                           (double *)dx, (double *)dw, tol, mi, (int *)dit, (double *)((char *)dit + 8), e->stream));
     struct { int it; int pad; double rr; } hr;
+    HIPCHK(bicgstab_device(n, (const int64_t *)drp, (const int32_t *)dci, (const double *)dv, (const double *)db,
+                           (double *)dx, dw, tol, mi, 8, &hr.it, &hr.rr, e->stream));
     HIPCHK(hipMemcpyAsync(u, dx, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(&hr, dit, 16, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (iters) *iters = hr.it;
     if (relres) *relres = hr.rr;

@@ -45,10 +45,14 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 }
 
 // One work-group per work item: per-side RBF moments per sub-batch, G_beta = -Psi^-T Phi per
-// CSR column, sums in fixed order (sub-batch -> item), B sums per sub-batch.
+// CSR column, sums in fixed order (sub-batch -> item), B sums per sub-batch.  A group of kNBG
+// sub-batches is staged in LDS with, per (sub-batch, side), the list of its paths that exit
+// through that side in path order; the moment of entry e in sub-batch b then loops over that
+// list only, adding in path order (the order of the per-exit loop it replaces).
 __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     extern __shared__ double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
+    __shared__ int s_lcnt[kNBG * 4];
     __shared__ unsigned long long s_steps;
     __shared__ unsigned int s_cnt[8];
     const int tid = threadIdx.x;
@@ -61,9 +65,16 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     const double *rx = a.rec.x + g0, *ry = a.rec.y + g0, *rY = a.rec.Y + g0, *rs = a.rec.score + g0;
     const int8_t *rside = a.rec.side + g0;
     const int32_t *rst = a.rec.steps + g0;
+    const int S = a.S;
+    const int GS = kNBG * S;
     double *Phi = smem;
     double *accS = smem + kNBG * a.ne_cap;
     double *accQ = accS + a.nnz_cap;
+    double *gx = accQ + a.nnz_cap;
+    double *gy = gx + GS;
+    double *gY = gy + GS;
+    uint16_t *lst = (uint16_t *)(gY + GS);
+    int8_t *gside = (int8_t *)(lst + 4 * GS);
     if (tid == 0) {
         s_steps = 0ULL;
         for (int q = 0; q < 8; ++q) s_cnt[q] = 0u;
@@ -85,7 +96,6 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     }
     const int ne = sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3];
     const int nnz = sd.csr_nnz;
-    const int S = a.S;
     const int nsb = (it.n + S - 1) / S;
     int eoff[5];
     eoff[0] = 0;
@@ -94,26 +104,48 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     double Bs = 0.0, BBs = 0.0;
     for (int g = 0; g < nsb; g += kNBG) {
         const int nb = (nsb - g) < kNBG ? (nsb - g) : kNBG;
-        for (int e = tid; e < ne; e += WG) {
-            const int se = a.ent_side[sd.ent_off + e];
-            const double pxe = a.ent_x[sd.ent_off + e], pye = a.ent_y[sd.ent_off + e];
-            const double ic2 = sd.inv_c2[se];
-            for (int b = 0; b < nb; ++b) {
-                const int p0 = (g + b) * S;
-                const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
-                double acc = 0.0;
-                for (int p = p0; p < p1; ++p)
-                    if (rside[p] == se) acc = acc + rY[p] * rbf(pxe, pye, rx[p], ry[p], ic2);
-                Phi[b * a.ne_cap + e] = acc;
-            }
+        const int q0 = g * S;                                   // first path of the group
+        const int gn = (it.n - q0) < nb * S ? (it.n - q0) : nb * S;
+        for (int p = tid; p < gn; p += WG) {
+            gx[p] = rx[q0 + p];
+            gy[p] = ry[q0 + p];
+            gY[p] = rY[q0 + p];
+            gside[p] = rside[q0 + p];
         }
-        for (int b = tid; b < nb; b += WG) {
-            const int p0 = (g + b) * S;
+        __syncthreads();
+        // per (sub-batch, side) lists in path order, and the B sums of each sub-batch
+        if (tid < nb * 4) {
+            const int b = tid >> 2, q = tid & 3;
+            const int p0 = b * S, p1 = (p0 + S) < gn ? (p0 + S) : gn;
+            int c = 0;
+            for (int p = p0; p < p1; ++p)
+                if (gside[p] == q) lst[(b * 4 + q) * S + c++] = (uint16_t)(p - p0);
+            s_lcnt[b * 4 + q] = c;
+        } else if (tid >= 64 && tid < 64 + nb) {
+            const int b = tid - 64;
+            const int p0 = q0 + b * S;
             const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
             double bs = 0.0, bb = 0.0;
             for (int p = p0; p < p1; ++p) { const double v = rs[p]; bs = bs + v; bb = bb + v * v; }
             s_bsb[b] = bs;
             s_bbsb[b] = bb;
+        }
+        __syncthreads();
+        // moments: unit u = (sub-batch b, entry e), entries fastest so that a wave reads one list
+        for (int u = tid; u < nb * ne; u += WG) {
+            const int b = u / ne, e = u - b * ne;
+            const int se = a.ent_side[sd.ent_off + e];
+            const double pxe = a.ent_x[sd.ent_off + e], pye = a.ent_y[sd.ent_off + e];
+            const double ic2 = sd.inv_c2[se];
+            const uint16_t *L = lst + (b * 4 + se) * S;
+            const int nl = s_lcnt[b * 4 + se];
+            const int pb = b * S;
+            double acc = 0.0;
+            for (int l = 0; l < nl; ++l) {
+                const int p = pb + L[l];
+                acc = acc + gY[p] * rbf(pxe, pye, gx[p], gy[p], ic2);
+            }
+            Phi[b * a.ne_cap + e] = acc;
         }
         __syncthreads();
         for (int c = tid; c < nnz; c += WG) {
@@ -226,7 +258,8 @@ __global__ void lut_eval_kernel(LutView L, int field, int64_t n, const double *x
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double v[8];
-    if (L.stride == 8) lut_eval<double, 8>(L, x[i], y[i], v);
+    if (L.nf > 5) lut_eval<double, 8>(L, x[i], y[i], v);
+    else if (L.nf == 5) lut_eval<double, 5>(L, x[i], y[i], v);
     else lut_eval<double, 4>(L, x[i], y[i], v);
     out[i] = v[field];
 }

@@ -23,6 +23,21 @@
 #include "pdds_pde.h"
 #include "engine_types.h"
 
+// Waves per SIMD the path kernels are register-budgeted for.  FP64: the LUT kernel gets up to
+// 256 VGPRs (2 waves), so the LUT loads of a step are in flight together, and so does the
+// Monegros problem (three transcendental chains per step would spill at 168); the other
+// analytic problems keep 3 waves (up to 168 VGPRs).  FP32: 4 waves.
+#ifndef PDDS_F64_WAVES_WIDE
+#define PDDS_F64_WAVES_WIDE 2
+#endif
+#ifndef PDDS_F64_WAVES
+#define PDDS_F64_WAVES 3
+#endif
+template <class R, int PDE> constexpr int path_waves() {
+    if constexpr (sizeof(R) == 4) return 4;
+    else return (PDE == pdds::PDE_VARCOEF_LUT || PDE == pdds::PDE_MONEGROS) ? PDDS_F64_WAVES_WIDE : PDDS_F64_WAVES;
+}
+
 namespace pdds {
 
 // Exit classification of FP64 paths: global-boundary test of Reduce_Analytic
@@ -101,25 +116,34 @@ __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uin
         normals_f64(seed, stream, knot, s.path, s.st, a, b);
         n1 = a; n2 = b;
     } else {
-        if ((s.st & 1) == 0) {
-            int64_t call = s.st >> 1;
-            Philox4 w = philox4x32_10((uint32_t)call, s.path, knot, (uint32_t)(call >> 32), seed, stream);
-            const float sc = 5.9604644775390625e-08f;
-            float u1 = ((float)(w.v[0] >> 8) + 0.5f) * sc;
-            float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
-            float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
-            float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
-            float r1 = __builtin_sqrtf(-2.0f * RM<float>::log(u1));
-            float r2 = __builtin_sqrtf(-2.0f * RM<float>::log(u3));
-            n1 = r1 * __builtin_amdgcn_cosf(u2);
-            n2 = r1 * __builtin_amdgcn_sinf(u2);
-            s.pn1 = r2 * __builtin_amdgcn_cosf(u4);
-            s.pn2 = r2 * __builtin_amdgcn_sinf(u4);
-        } else {
-            n1 = s.pn1;
-            n2 = s.pn2;
-        }
+        // one Philox call per even step gives the normals of steps st and st + 1; computed
+        // for every lane and selected (lanes of one wave rarely share the step parity, so a
+        // branch would run both sides anyway, and it would split the straight-line step)
+        const bool even = (s.st & 1) == 0;
+        const int64_t call = s.st >> 1;
+        Philox4 w = philox4x32_10((uint32_t)call, s.path, knot, (uint32_t)(call >> 32), seed, stream);
+        const float sc = 5.9604644775390625e-08f;
+        float u1 = ((float)(w.v[0] >> 8) + 0.5f) * sc;
+        float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
+        float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
+        float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
+        float r1 = __builtin_sqrtf(-2.0f * RM<float>::log(u1));
+        float r2 = __builtin_sqrtf(-2.0f * RM<float>::log(u3));
+        const float a1 = r1 * __builtin_amdgcn_cosf(u2), a2 = r1 * __builtin_amdgcn_sinf(u2);
+        const float b1 = r2 * __builtin_amdgcn_cosf(u4), b2 = r2 * __builtin_amdgcn_sinf(u4);
+        n1 = even ? a1 : s.pn1;
+        n2 = even ? a2 : s.pn2;
+        s.pn1 = even ? b1 : s.pn1;
+        s.pn2 = even ? b2 : s.pn2;
     }
+}
+
+// Lane 0's value in every lane, held in scalar registers: the queue bookkeeping of a wave
+// (pool bounds, queue index) is uniform and does not need vector registers.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 // Knot of global path g: the last knot whose first path is <= g.
@@ -137,11 +161,12 @@ __device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int
 // Path kernel
 // ----------------------------------------------------------------------------
 template <class R, int PDE, bool PERSIST>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 4))) path_kernel(PathKernelArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
     const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
     const int lane = __lane_id();
     PathState<R> s;
-    Coef<R> co;
+    Coef<R> co;            // coefficients at the current X
+    R pn1 = 0, pn2 = 0;    // normals of step s.st (the next step)
     R rect[4];
     double rectd[4];
     uint32_t kid = 0;
@@ -171,21 +196,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         else done = true;
     }
     // wave-private pool of consecutive paths [pool_next, pool_end), refilled kPoolChunk
-    // paths at a time from the device-wide counter (one atomic per kPoolChunk paths)
+    // paths at a time from a path queue (one atomic per kPoolChunk paths).  The wave starts
+    // on its XCD's queue, so the paths (and LUT texels) of one XCD's L2 stay in one
+    // contiguous range of knots; a drained queue sends the wave on to the next one.
     int64_t pool_next = 0, pool_end = 0;
     bool drained = false;
+    int q = PERSIST ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
+    int tried = 0;
     while (true) {
         if (PERSIST) {
             const bool need = !active && !done;
             const unsigned long long m = __ballot(need);
             if (m) {
-                if (pool_next >= pool_end && !drained) {
+                while (pool_next >= pool_end && !drained) {
                     unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)kPoolChunk);
-                    base = __shfl(base, 0);
-                    pool_next = (int64_t)base;
-                    pool_end = pool_next + kPoolChunk < a.total ? pool_next + kPoolChunk : a.total;
-                    if (pool_next >= a.total) drained = true;
+                    if (lane == 0) base = atomicAdd(a.counter + q, (unsigned long long)kPoolChunk);
+                    base = uniform_u64(base);
+                    const int64_t qe = a.total * (q + 1) / a.n_queues;
+                    const int64_t s0 = a.total * q / a.n_queues + (int64_t)base;
+                    if (s0 < qe) {
+                        pool_next = s0;
+                        pool_end = s0 + kPoolChunk < qe ? s0 + kPoolChunk : qe;
+                    } else if (++tried >= a.n_queues) {
+                        drained = true;
+                    } else {
+                        q = q + 1 == a.n_queues ? 0 : q + 1;
+                    }
                 }
                 const int64_t avail = pool_end > pool_next ? pool_end - pool_next : 0;
                 const int rank = __popcll(m & ((1ULL << lane) - 1ULL));
@@ -198,41 +234,54 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
             }
         }
         if (!__any(active)) break;
-        R d = 0, ex = 0, ey = 0, nx = 0, ny = 0;
-        int kind = 0;
-        if (active && !fresh) {
-            R n1, n2;
-            draw_normals<R>(s, a.seed, a.stream, kid, n1, n2);
-            em_step<R>(s, co, h, sqrth, n1, n2);
-            s.st += 1;
-            d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        }
-        if (active) pde_coef<PDE, R>(s.x, s.y, a.pp, co);
-        if (active && !fresh) {
-            R ga = nx * co.s00 + ny * co.s10;
-            R gb = nx * co.s01 + ny * co.s11;
-            R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-            const bool go = d < beta * nrm * sqrth;
-            const bool capped = s.st >= a.max_steps;
-            if (!go || capped) {
-                int side;
-                if constexpr (sizeof(R) == 8) side = classify_coord(rectd, a.glob, (double)ex, (double)ey);
-                else side = classify_kind(kind, gmask);
-                double sc;
-                if (side == 4) {
-                    R gv = pde_g<PDE, R>(ex, ey, a.pp);
-                    sc = (double)(s.Z + s.Y * gv + s.xi);
-                } else {
-                    sc = (double)(s.Z + s.xi);
-                }
-                a.rec.x[g] = (double)ex;
-                a.rec.y[g] = (double)ey;
-                a.rec.Y[g] = (double)s.Y;
-                a.rec.score[g] = sc;
-                a.rec.side[g] = (int8_t)side;
-                a.rec.steps[g] = (int32_t)s.st | ((capped && go) ? kCappedBit : 0);
-                active = false;
+        // The body below is one straight-line block for every lane (a lane that is not
+        // stepping computes and discards), so the LUT loads of the new X are issued
+        // together and overlap the next step's normal draw, which does not depend on X.
+        // Inactive lanes hold stale or unset state: they touch no memory except the LUT,
+        // whose coordinates are clamped into the table.
+        const bool stepping = active && !fresh;
+        {
+            PathState<R> t = s;
+            em_step<R>(t, co, h, sqrth, pn1, pn2);
+            s.x = stepping ? t.x : s.x;
+            s.y = stepping ? t.y : s.y;
+            s.Y = stepping ? t.Y : s.Y;
+            s.Z = stepping ? t.Z : s.Z;
+            s.xi = stepping ? t.xi : s.xi;
+            if constexpr (sizeof(R) == 4) {
+                s.zc = stepping ? t.zc : s.zc;
+                s.xc = stepping ? t.xc : s.xc;
             }
+            s.st += stepping ? 1 : 0;
+        }
+        R ex, ey, nx, ny;
+        int kind;
+        const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        pde_coef<PDE, R>(s.x, s.y, a.pp, co);
+        draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
+        const R ga = nx * co.s00 + ny * co.s10;
+        const R gb = nx * co.s01 + ny * co.s11;
+        const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
+        const bool go = d < beta * nrm * sqrth;
+        const bool capped = s.st >= a.max_steps;
+        if (stepping && (!go || capped)) {
+            int side;
+            if constexpr (sizeof(R) == 8) side = classify_coord(rectd, a.glob, (double)ex, (double)ey);
+            else side = classify_kind(kind, gmask);
+            double sc;
+            if (side == 4) {
+                R gv = pde_g<PDE, R>(ex, ey, a.pp);
+                sc = (double)(s.Z + s.Y * gv + s.xi);
+            } else {
+                sc = (double)(s.Z + s.xi);
+            }
+            a.rec.x[g] = (double)ex;
+            a.rec.y[g] = (double)ey;
+            a.rec.Y[g] = (double)s.Y;
+            a.rec.score[g] = sc;
+            a.rec.side[g] = (int8_t)side;
+            a.rec.steps[g] = (int32_t)s.st | ((capped && go) ? kCappedBit : 0);
+            active = false;
         }
         fresh = false;
     }

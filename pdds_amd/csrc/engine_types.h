@@ -6,8 +6,19 @@
 namespace pdds {
 
 constexpr int kNBG = 8;                    // sub-batches reduced per pass over Psi^-1
+
+// LDS of the reduction kernel: Phi [kNBG][ne_cap], accS/accQ [nnz_cap] (FP64), the group's
+// records x, y, Y [kNBG*S] (FP64), their sides [kNBG*S] (int8) and per-(sub-batch, side)
+// path lists [kNBG][4][S] (uint16)
+inline size_t reduce_lds_bytes(int ne_cap, int nnz_cap, int S) {
+    const size_t g = (size_t)kNBG * S;
+    size_t b = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap + 3 * g) * sizeof(double);
+    b += g * 4 * sizeof(uint16_t) + g;
+    return (b + 7) & ~(size_t)7;
+}
 constexpr int32_t kCappedBit = 1 << 30;    // record flag: path stopped by the step cap
-constexpr int64_t kPoolChunk = 256;        // paths a wave takes from the device-wide queue at once
+constexpr int64_t kPoolChunk = 256;        // paths a wave takes from a path queue at once
+constexpr int kMaxQueues = 8;              // path queues: one per XCD of an MI355X (8 XCDs x 32 CUs)
 
 // One Monte Carlo knot of an assembly batch.
 struct KnotDesc {
@@ -69,7 +80,11 @@ struct PathKernelArgs {
     const StencilDesc *sten;
     int64_t total;          // paths in the batch
     int64_t base;           // plain launches: first path of this launch
+    // persistent launches: queue q serves paths [q*total/n_queues, (q+1)*total/n_queues)
+    // through counter[q]; work-group b starts on queue b % n_queues (the XCD it is
+    // dispatched to) and moves to the next queue when its own is drained
     unsigned long long *counter;
+    int32_t n_queues;
     Records rec;
     double glob[4];
     double h, sqrth, beta;

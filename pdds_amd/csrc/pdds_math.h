@@ -306,17 +306,31 @@ PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kin
 }
 
 // --------------------------------------------------------------- LUT -----
-// Cubic B-spline coefficient table [ny][nx][stride] (FP32, stride 4 or 8 floats per texel,
-// 16-byte aligned), t = (x - ox) * sx.  Like the reference's FP32 texture lookups
-// (Integrators/cubicTex2D.cu:64-87), the 16-tap weighted sum runs in FP32 -- here with
-// explicit fma -- whatever the path precision; only the coordinate is formed in R.
+// Cubic B-spline coefficient table of nx x ny texels, t = (x - ox) * sx.  Like the
+// reference's FP32 texture lookups (Integrators/cubicTex2D.cu:64-87), the 16-tap weighted
+// sum runs in FP32 -- here with explicit fma -- whatever the path precision; only the
+// coordinate is formed in R.
+//
+// Device layout: the table is stored with kLutPad texels of replicated border on the
+// low side and kLutPad + 1 on the high side (padded row of px = nx + 2*kLutPad + 1
+// texels), so every tap of a clamped coordinate t in [-2, nx + 1] lies inside the
+// padded table and no per-tap clamp is needed; the replicated border gives exactly the
+// clamp-to-edge values.  Fields 0..3 live in one float4 plane (a).  For nf == 5, plane b
+// holds field 4 as x-quads: entry t = (f4[t], f4[t+1], f4[t+2], f4[t+3]) along the padded
+// row, so the four taps of a row are one 16-byte load; for nf > 5 plane b holds fields
+// 4..7 per texel.  Every load is at a 32-bit byte offset from a uniform base.  A gather of
+// scattered lanes costs one cache line per lane and instruction, so a step costs 16 + 4
+// line accesses per lane for nf == 5 (16 + 16 for nf > 5).
+constexpr int kLutPad = 3;
+
 struct LutView {
-    const float *coef;
-    int nx, ny, stride, nf;
+    const float4 *a;       // fields 0..3
+    const float4 *b;       // nf == 5: x-quads of field 4; nf > 5: fields 4..7; else null
+    int nx, ny, px, nf;    // logical texels per row / column, padded row length, fields
     double ox, oy, sx, sy;
 };
 
-// Uniform cubic B-spline weights of taps -1..2 at fraction f (Horner forms, fma)
+// Cubic B-spline weights at fraction f in FP32 (explicit fma)
 PDDS_HD void bspline_wf(float f, float w[4]) {
     const float s6 = 1.0f / 6.0f;
     const float f2 = f * f, om = 1.0f - f;
@@ -326,7 +340,12 @@ PDDS_HD void bspline_wf(float f, float w[4]) {
     w[3] = (f2 * f) * s6;
 }
 
-// Evaluate fields [0, NF) (NF <= stride) at (x,y):
+template <class T>
+PDDS_HD const T &lut_at(const void *base, uint32_t byte_off) {
+    return *(const T *)((const char *)base + byte_off);
+}
+
+// Evaluate fields [0, NF) (NF <= 8) at (x,y):
 //   sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
 template <class R, int NF>
 PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
@@ -339,27 +358,30 @@ PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
     float wx[4], wy[4];
     bspline_wf((float)(t - fi), wx);
     bspline_wf((float)(v - fj), wy);
-    const int i = (int)fi, j = (int)fj;
+    // padded index of tap (a=0, b=0): (j - 1 + kLutPad) * px + (i - 1 + kLutPad), >= 0
+    const uint32_t t0 = (uint32_t)((int)fj + (kLutPad - 1)) * (uint32_t)L.px + (uint32_t)((int)fi + (kLutPad - 1));
     float acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        int jj = j - 1 + b;
-        jj = jj < 0 ? 0 : (jj > L.ny - 1 ? L.ny - 1 : jj);
+        const uint32_t r0 = t0 + (uint32_t)b * (uint32_t)L.px;
         float row[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) row[f] = 0.0f;
+        float4 quad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (NF == 5) quad = lut_at<float4>(L.b, r0 * 16u);
+        const float f4[4] = {quad.x, quad.y, quad.z, quad.w};
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            int ii = i - 1 + a;
-            ii = ii < 0 ? 0 : (ii > L.nx - 1 ? L.nx - 1 : ii);
-            const float4 *cp = (const float4 *)(L.coef + ((int64_t)jj * L.nx + ii) * L.stride);
+            const uint32_t ti = r0 + (uint32_t)a;
             float c[8];
-            const float4 q0 = cp[0];
+            const float4 q0 = lut_at<float4>(L.a, ti * 16u);
             c[0] = q0.x; c[1] = q0.y; c[2] = q0.z; c[3] = q0.w;
-            if constexpr (NF > 4) {
-                const float4 q1 = cp[1];
+            if constexpr (NF == 5) {
+                c[4] = f4[a];
+            } else if constexpr (NF > 5) {
+                const float4 q1 = lut_at<float4>(L.b, ti * 16u);
                 c[4] = q1.x; c[5] = q1.y; c[6] = q1.z; c[7] = q1.w;
             }
 #pragma unroll

@@ -10,10 +10,13 @@ namespace pdds {
 // second difference at each end node to the data's second difference extrapolated to that node.
 void lut_prefilter(int nx, int ny, int nf, const double *samples, float *coef);
 
-// BiCGSTAB (van der Vorst 1992) for (I + G) u = B with G in CSR, zero initial guess, run by one
-// resident work-group so the whole iteration stays on the device with fixed-order reductions.
+// BiCGSTAB (van der Vorst 1992) for (I + G) u = B with G in CSR, zero initial guess, run on the
+// device by three kernels per iteration with fixed-order reductions; the host polls a status word
+// every `batch` iterations.  work must hold bicgstab_work_bytes(n, maxit) bytes.  A breakdown
+// (rh.r ~ 0) restarts from the current iterate.
+size_t bicgstab_work_bytes(int n, int maxit);
 hipError_t bicgstab_device(int n, const int64_t *row_ptr, const int32_t *cols, const double *vals, const double *b,
-                           double *x, double *work, double tol, int maxit, int *iters_dev, double *relres_dev,
+                           double *x, void *work, double tol, int maxit, int batch, int *iters, double *relres,
                            hipStream_t s);
 
 }  // namespace pdds

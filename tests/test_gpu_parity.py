@@ -119,6 +119,28 @@ def test_solve_knot_bit_exact(eng, knot, quirk):
     assert got["counters"]["exits"] == [int(v) for v in ref["cnt"]]
 
 
+@pytest.mark.parametrize("plain", [0, 1], ids=["persistent", "plain"])
+def test_solve_knot_lut_bit_exact(eng, plain):
+    """The assembly path kernel on the coefficient-LUT problem (the benchmark's kernel), persistent
+    queue and plain launch: engine == oracle bit for bit."""
+    from pdds_amd.engine import bvp, default_run
+    from pdds_amd.workloads import relative_fac
+    lut, olut = _lut_pair(eng)
+    fac = relative_fac((-1.0, -1.0), (1.0, 1.0), 4, 8, 1.0)
+    m = _product_mesh((-1.0, -1.0), (1.0, 1.0), 4, 8, fac)
+    knot, n = 43, 3000                  # four non-empty stencil sides
+    st = m.stencil(knot)
+    run = default_run(h=1e-3, seed=5, sub_batch=32, item_paths=640, plain=plain)
+    got = eng.solve_knot(m, bvp("varcoef_lut", lut=lut), run, knot, n)
+    pb = O.problem(O.PDE_VARCOEF_LUT, lut=olut)
+    paths = O.simulate(pb, O.run_params(1e-3, 5, threads=8), m.x0[knot], m.y0[knot], st["rect"], knot, 0, n)
+    ref = O.reduce_batched(pb, paths, st, m.glob, _cshape(st, fac), st["ipsi"], 32, 640, quirk=1)
+    assert np.array_equal(got["G"], ref["G"])
+    assert np.array_equal(got["Gvar"], ref["Gvar"])
+    assert got["B"] == ref["B"] and got["BB"] == ref["BB"]
+    assert got["counters"]["path_steps"] == int(paths["steps"].sum())
+
+
 def test_assemble_fixture_statistical_parity(eng, golden):
     """Whole fixture run (165 knots, N=5000, h=0.005) against the reference's recorded G/B at 3 SE."""
     from pdds_amd.engine import bvp, default_run
@@ -201,6 +223,40 @@ def test_device_bicgstab_matches_dense(eng):
     assert np.abs(u - ud).max() < 1e-10 * np.abs(ud).max()
     uo, ito, rro = O.bicgstab(np.asarray(m.row_ptr), m.cols, vals, B, tol=1e-13)
     assert np.abs(uo - ud).max() < 1e-10 * np.abs(ud).max()
+    # same iteration as the oracle up to the order of the dot-product sums
+    assert abs(it - ito) <= 2
+
+
+def test_device_bicgstab_zero_rhs(eng):
+    m = _product_mesh()
+    vals = np.full(m.nnz, -0.01)
+    u, it, rr = eng.solve(m.n_knots, m.row_ptr, m.cols, vals, np.zeros(m.n_knots), tol=1e-12)
+    assert it == 0 and rr == 0.0 and not np.any(u)
+
+
+def test_device_bicgstab_large_mesh(eng):
+    """The 8x8 x 32-knot mesh (3437 knots, ~8.7e5 entries): G with the sign and row sums of an
+    interface system (G <= 0, row sums in (-1, 0)), against a dense solve, twice in a row (the
+    second call reuses the device buffers)."""
+    from pdds_amd import workloads as W
+    from pdds_amd.engine import Mesh
+    sw, ne, ns, nn, _, _, _, _, shape = W.CONFIGS["cfg2_varcoef_lut"]
+    m = Mesh(sw, ne, ns, nn, W.relative_fac(sw, ne, ns, nn, shape))
+    rng = np.random.default_rng(11)
+    vals = -rng.uniform(0.0, 1.0, m.nnz)
+    for k in range(m.n_knots):
+        a, b = m.row_ptr[k], m.row_ptr[k + 1]
+        if b > a:
+            vals[a:b] *= rng.uniform(0.2, 0.9) / vals[a:b].sum() * -1.0
+    A = np.eye(m.n_knots)
+    for k in range(m.n_knots):
+        A[k, m.row_cols(k)] += vals[m.row_ptr[k]:m.row_ptr[k + 1]]
+    for seed in (1, 2):
+        B = np.random.default_rng(seed).normal(3, 0.5, m.n_knots)
+        u, it, rr = eng.solve(m.n_knots, m.row_ptr, m.cols, vals, B, tol=1e-12)
+        ud = np.linalg.solve(A, B)
+        assert rr < 1e-12 and it > 0
+        assert np.abs(u - ud).max() < 1e-9 * np.abs(ud).max()
 
 
 def test_lut_device_eval_matches_oracle(eng):

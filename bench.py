@@ -179,10 +179,12 @@ def main():
         "device": info,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "path_kernel<double, PDE_VARCOEF_LUT, persistent>",
+                     "kernel": ("path_kernel_tiled<double> (LDS-tiled LUT)" if c["wg_size"] != 256
+                                else "path_kernel<double, PDE_VARCOEF_LUT, persistent>"),
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
-                     "note": "FP64 VALU-bound (no dense contraction); peak = MI355X FP64 vector = FP64 matrix "
-                             "78.6 TFLOP/s; achieved counts %g algorithmic flops per path-step"
+                     "note": "no dense contraction: peak = MI355X FP64 vector (= FP64 matrix) 78.6 TFLOP/s; "
+                             "achieved counts %g algorithmic flops per path-step; traffic = HBM bytes per launch "
+                             "from FETCH_SIZE/WRITE_SIZE (copy-calibrated), mostly scattered exit-record stores"
                              % FLOPS_PER_STEP.get(args.config, 302.0)},
     }
     if not args.no_extras:

@@ -2,9 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -58,6 +60,7 @@ struct pdds_engine {
     int tables_next = 0;
     int64_t record_budget = (int64_t)24 << 30;   // bytes of exit records per assembly batch
     int path_wgs_per_cu = 8;                      // persistent path-kernel work-groups per CU
+    size_t lds_max = 64 * 1024;                   // LDS bytes a work-group may use
 };
 
 struct pdds_lut {
@@ -85,7 +88,7 @@ namespace {
 
 enum BufSlot {
     B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_RECX, B_RECY,
-    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
+    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_PATCH, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
     B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
 };
 
@@ -213,6 +216,25 @@ pdds_status build_tables(pdds_engine *e, const std::vector<StencilDesc> &sten, c
     return PDDS_OK;
 }
 
+// Plane-a texels a path inside rectangle rect can touch, one texel of slack on each side for
+// the rounding of the coordinate map: padded columns [c0, c0 + tw), rows [r0, r0 + th).
+void lut_tile_of(const LutView &L, const double rect[4], int &c0, int &r0, int &tw, int &th) {
+    auto span = [](double a, double b, double o, double sc, int n, int np, int &lo, int &len) {
+        auto idx = [&](double x) {
+            double t = (x - o) * sc;
+            t = std::min(std::max(t, -2.0), (double)n + 1.0);
+            return (int)std::floor(t);
+        };
+        const int i0 = std::max(idx(a) - 1, -2), i1 = std::min(idx(b) + 1, n + 1);
+        lo = std::max(i0 + kLutPad - 1, 0);
+        const int hi = std::min(i1 + kLutPad + 2, np - 1);
+        len = hi - lo + 1;
+    };
+    const int py = L.ny + 2 * kLutPad + 1;
+    span(rect[0], rect[2], L.ox, L.sx, L.nx, L.px, c0, tw);
+    span(rect[1], rect[3], L.oy, L.sy, L.ny, py, r0, th);
+}
+
 pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
                          const DevTables &T, const std::vector<AsmKnot> &knots, int64_t n_vals,
                          double *G_host, double *Gvar_host, AsmOut &out) {
@@ -231,6 +253,29 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     out.wg = wg;
     out.kstats.assign(knots.size(), KnotStat{});
     if (knots.empty()) return PDDS_OK;
+    // LDS-tiled form (coefficient LUT with a fifth-field quad plane, persistent launch): knots
+    // are grouped into patches of equal tile, tiles that fit in LDS first, in first-seen order
+    const bool tiled = persistent && prob->pde == PDDS_PDE_VARCOEF_LUT && r->lut_lds != 0 && pp.lut.nf == 5;
+    const size_t tile_cap = e->lds_max > 1024 ? e->lds_max - 1024 : 0;
+    std::vector<size_t> ord(knots.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+    std::vector<std::array<int, 5>> tkey(knots.size());   // c0, r0, tw, th, fits
+    if (tiled) {
+        std::map<std::array<int, 4>, int> first;
+        std::vector<int> rank(knots.size());
+        for (size_t i = 0; i < knots.size(); ++i) {
+            int c0, r0, tw, th;
+            lut_tile_of(pp.lut, sten[knots[i].stencil].rect, c0, r0, tw, th);
+            const int fits = (size_t)tw * th * sizeof(float4) <= tile_cap ? 1 : 0;
+            tkey[i] = {c0, r0, tw, th, fits};
+            const std::array<int, 4> k4 = fits ? std::array<int, 4>{c0, r0, tw, th} : std::array<int, 4>{-1, -1, -1, -1};
+            auto it = first.find(k4);
+            if (it == first.end()) it = first.emplace(k4, fits ? (int)i : (int)knots.size() + (int)i).first;
+            rank[i] = it->second;
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return rank[a] < rank[b]; });
+        out.wg = path_tiled_wg(r->precision);
+    }
     // knots are processed in batches whose exit records fit the record budget
     const int64_t rec_bytes = 4 * 8 + 1 + 4;
     const int64_t budget = e->record_budget / rec_bytes;
@@ -246,14 +291,14 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         // batch [k0, k1)
         size_t k1 = k0;
         int64_t tot = 0;
-        while (k1 < knots.size() && (k1 == k0 || tot + knots[k1].n <= budget)) tot += knots[k1++].n;
+        while (k1 < knots.size() && (k1 == k0 || tot + knots[ord[k1]].n <= budget)) tot += knots[ord[k1++]].n;
         const int nk = (int)(k1 - k0);
         std::vector<KnotDesc> kd(nk);
         std::vector<ItemDesc> items;
         int64_t part = 0, off = 0;
         int ne_cap = 1, nnz_cap = 1;
         for (int q = 0; q < nk; ++q) {
-            const AsmKnot &k = knots[k0 + q];
+            const AsmKnot &k = knots[ord[k0 + q]];
             const StencilDesc &sd = sten[k.stencil];
             const int P = r->item_paths > 0 ? r->item_paths : choose_P(k.n, S);
             Pmax_all = std::max(Pmax_all, P);
@@ -292,8 +337,37 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st; dstats = (ItemStat *)p;
         if ((st = dbuf(e, B_PART, (size_t)std::max<int64_t>(part, 1) * sizeof(double), &p))) return st; dpart = (double *)p;
         if ((st = dbuf(e, B_KSTAT, (size_t)nk * sizeof(KnotStat), &p))) return st; dks = (KnotStat *)p;
-        if ((st = dbuf(e, B_COUNTER, kMaxQueues * sizeof(unsigned long long), &p))) return st; dctr = (unsigned long long *)p;
-        HIPCHK(hipMemsetAsync(dctr, 0, kMaxQueues * sizeof(unsigned long long), e->stream));
+        // patches of the batch: runs of equal tile among the ordered knots
+        std::vector<PatchDesc> patches;
+        size_t lds_bytes = 0;
+        if (tiled) {
+            int64_t poff = 0;
+            for (int q = 0; q < nk; ++q) {
+                const std::array<int, 5> &tk = tkey[ord[k0 + q]];
+                const bool same = !patches.empty() &&
+                                  (tk[4] ? (patches.back().tiled && patches.back().c0 == tk[0] && patches.back().r0 == tk[1] &&
+                                            patches.back().tw == tk[2] && patches.back().th == tk[3])
+                                         : !patches.back().tiled);
+                if (!same) {
+                    PatchDesc pd;
+                    std::memset(&pd, 0, sizeof(pd));
+                    pd.pbeg = poff;
+                    pd.tiled = tk[4];
+                    if (tk[4]) {
+                        pd.c0 = tk[0]; pd.r0 = tk[1]; pd.tw = tk[2]; pd.th = tk[3];
+                        lds_bytes = std::max(lds_bytes, (size_t)tk[2] * tk[3] * sizeof(float4));
+                    }
+                    patches.push_back(pd);
+                }
+                poff += kd[q].n_paths;
+                patches.back().pend = poff;
+            }
+        }
+        const size_t nctr = std::max<size_t>(kMaxQueues, patches.size());
+        PatchDesc *dpatch = nullptr;
+        if (tiled && (st = upload(e, B_PATCH, patches, &dpatch))) return st;
+        if ((st = dbuf(e, B_COUNTER, nctr * sizeof(unsigned long long), &p))) return st; dctr = (unsigned long long *)p;
+        HIPCHK(hipMemsetAsync(dctr, 0, nctr * sizeof(unsigned long long), e->stream));
         // path kernel
         PathKernelArgs pa;
         std::memset(&pa, 0, sizeof(pa));
@@ -309,14 +383,26 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             grid = (int)std::min<int64_t>((tot + wg - 1) / wg, (int64_t)1 << 30);
         }
         pa.n_queues = persistent ? std::min(kMaxQueues, grid) : 1;
+        pa.patches = dpatch;
+        pa.n_patches = (int32_t)patches.size();
+        const int twg = path_tiled_wg(r->precision);
+        if (tiled) {
+            grid = r->grid_wgs > 0 ? r->grid_wgs
+                                   : (int)std::max<int64_t>(1, std::min<int64_t>(e->n_cu, (tot + 8 * twg - 1) / (8 * twg)));
+        }
         grid_used = std::max(grid_used, grid);
         HIPCHK(hipEventRecord(e->ev[0], e->stream));
         const int64_t per_launch = persistent ? tot : (int64_t)grid * wg;
         for (int64_t base = 0; base < tot; base += per_launch) {
             pa.base = base;
             const int g = persistent ? grid : (int)std::min<int64_t>(grid, (tot - base + wg - 1) / wg);
-            hipError_t lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, g, wg, e->stream, pa)
-                                                      : launch_path_f32(prob->pde, persistent, g, wg, e->stream, pa);
+            hipError_t lr;
+            if (tiled)
+                lr = r->precision == PDDS_FP64 ? launch_path_tiled_f64(g, lds_bytes, e->stream, pa)
+                                               : launch_path_tiled_f32(g, lds_bytes, e->stream, pa);
+            else
+                lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, g, wg, e->stream, pa)
+                                               : launch_path_f32(prob->pde, persistent, g, wg, e->stream, pa);
             if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("path launch: ") + hipGetErrorString(lr));
         }
         HIPCHK(hipEventRecord(e->ev[1], e->stream));
@@ -338,8 +424,10 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         hipError_t lr = launch_finalize(nk, 256, e->stream, f);
         if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
         HIPCHK(hipEventRecord(e->ev[2], e->stream));
-        HIPCHK(hipMemcpyAsync(out.kstats.data() + k0, dks, (size_t)nk * sizeof(KnotStat), hipMemcpyDeviceToHost, e->stream));
+        std::vector<KnotStat> ks(nk);
+        HIPCHK(hipMemcpyAsync(ks.data(), dks, (size_t)nk * sizeof(KnotStat), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
+        for (int q = 0; q < nk; ++q) out.kstats[ord[k0 + q]] = ks[q];
         float ms1 = 0, ms2 = 0;
         HIPCHK(hipEventElapsedTime(&ms1, e->ev[0], e->ev[1]));
         HIPCHK(hipEventElapsedTime(&ms2, e->ev[1], e->ev[2]));
@@ -479,6 +567,9 @@ pdds_status pdds_engine_create(int device, pdds_engine **out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         e->n_cu = prop.multiProcessorCount;
         e->name = prop.gcnArchName;
+        e->lds_max = std::max<size_t>({(size_t)prop.sharedMemPerBlock, (size_t)prop.maxSharedMemoryPerMultiProcessor,
+                                        (size_t)64 * 1024});
+        e->lds_max = std::min<size_t>(e->lds_max, (size_t)160 * 1024);
     }
     if (e->n_cu <= 0) e->n_cu = 256;
     *out = e;

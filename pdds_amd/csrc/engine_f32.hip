@@ -25,6 +25,16 @@ hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream
     }
 }
 
+hipError_t launch_path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(path_kernel_tiled<float>, dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int PDE>
 static hipError_t paths_f32(int grid, int wg, hipStream_t s, const PathsArgs &a) {
     hipLaunchKernelGGL((paths_kernel<float, PDE>), dim3(grid), dim3(wg), 0, s, a);

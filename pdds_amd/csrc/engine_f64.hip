@@ -3,6 +3,7 @@
 // with -ffp-contract=off so that the only fused multiply-adds are the explicit
 // fma() calls: every FP64 path and every reduction is then bit-identical to the
 // FP64 CPU restatement on the same Philox stream.
+#include "../../include/pdds_engine.h"
 #include "engine_kernels.h"
 
 namespace pdds {
@@ -25,6 +26,18 @@ hipError_t launch_path_f64(int pde, bool persistent, int grid, int wg, hipStream
         case PDE_HELMHOLTZ: return path_f64<PDE_HELMHOLTZ>(persistent, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
+}
+
+int path_tiled_wg(int precision) { return precision == PDDS_FP64 ? tiled_wg<double>() : tiled_wg<float>(); }
+
+hipError_t launch_path_tiled_f64(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<double>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(path_kernel_tiled<double>, dim3(grid), dim3(tiled_wg<double>()), lds, s, a);
+    return hipGetLastError();
 }
 
 template <int PDE>

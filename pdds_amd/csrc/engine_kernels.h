@@ -23,19 +23,16 @@
 #include "pdds_pde.h"
 #include "engine_types.h"
 
-// Waves per SIMD the path kernels are register-budgeted for.  FP64: the LUT kernel gets up to
-// 256 VGPRs (2 waves), so the LUT loads of a step are in flight together, and so does the
-// Monegros problem (three transcendental chains per step would spill at 168); the other
-// analytic problems keep 3 waves (up to 168 VGPRs).  FP32: 4 waves.
-#ifndef PDDS_F64_WAVES_WIDE
-#define PDDS_F64_WAVES_WIDE 2
-#endif
-#ifndef PDDS_F64_WAVES
-#define PDDS_F64_WAVES 3
-#endif
+// Waves per SIMD the path kernels are register-budgeted for (a wave gets up to 512 / waves
+// VGPRs).  FP64: the LUT kernels keep a step's LUT reads in flight with up to 256 VGPRs (2
+// waves), as does the Monegros problem (three transcendental chains per step); the other
+// variable-coefficient problem fits 3 waves and the Poisson/Helmholtz problems 4.  FP32: 3
+// waves for the LUT problem, 4 for the others.
 template <class R, int PDE> constexpr int path_waves() {
-    if constexpr (sizeof(R) == 4) return 4;
-    else return (PDE == pdds::PDE_VARCOEF_LUT || PDE == pdds::PDE_MONEGROS) ? PDDS_F64_WAVES_WIDE : PDDS_F64_WAVES;
+    if constexpr (sizeof(R) == 4) return PDE == pdds::PDE_VARCOEF_LUT ? 3 : 4;
+    else if constexpr (PDE == pdds::PDE_VARCOEF_LUT || PDE == pdds::PDE_MONEGROS) return 2;
+    else if constexpr (PDE == pdds::PDE_VARCOEF) return 3;
+    else return 4;
 }
 
 namespace pdds {
@@ -158,27 +155,29 @@ __device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int
 }
 
 // ----------------------------------------------------------------------------
-// Path kernel
+// Per-lane walker: the state of the path a lane carries and one step of it
 // ----------------------------------------------------------------------------
-template <class R, int PDE, bool PERSIST>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
-    const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
-    const int lane = __lane_id();
+template <class R, int PDE>
+struct Walker {
     PathState<R> s;
     Coef<R> co;            // coefficients at the current X
-    R pn1 = 0, pn2 = 0;    // normals of step s.st (the next step)
-    R rect[4];
-    double rectd[4];
-    uint32_t kid = 0;
-    int gmask = 0;
-    int64_t g = -1;
-    bool active = false, done = false, fresh = false;
+    R pn1, pn2;            // normals of step s.st (the next step)
+    R rect[4];             // stencil rectangle in path precision (FP64 classifies on it)
+    uint32_t kid;
+    int gmask;
+    int64_t g;             // batch path index (record slot)
+    bool active, fresh;
 
-    auto start = [&](int64_t gg) {
-        const int k = find_knot(a.knots, a.n_knots, gg);
+    __device__ __forceinline__ void init() {
+        pn1 = 0; pn2 = 0; kid = 0; gmask = 0; g = -1;
+        active = false; fresh = false;
+    }
+
+    // take path gg of the batch; k is its knot
+    __device__ __forceinline__ void start(const PathKernelArgs &a, int64_t gg, int k) {
         const KnotDesc &kd = a.knots[k];
         const StencilDesc &sd = a.sten[kd.stencil];
-        for (int q = 0; q < 4; ++q) { rectd[q] = sd.rect[q]; rect[q] = (R)sd.rect[q]; }
+        for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
         gmask = sd.gmask;
         kid = kd.knot_id;
         g = gg;
@@ -188,58 +187,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
         s.st = 0;
         active = true;
         fresh = true;
-    };
-
-    if (!PERSIST) {
-        const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (gg < a.total) start(gg);
-        else done = true;
     }
-    // wave-private pool of consecutive paths [pool_next, pool_end), refilled kPoolChunk
-    // paths at a time from a path queue (one atomic per kPoolChunk paths).  The wave starts
-    // on its XCD's queue, so the paths (and LUT texels) of one XCD's L2 stay in one
-    // contiguous range of knots; a drained queue sends the wave on to the next one.
-    int64_t pool_next = 0, pool_end = 0;
-    bool drained = false;
-    int q = PERSIST ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
-    int tried = 0;
-    while (true) {
-        if (PERSIST) {
-            const bool need = !active && !done;
-            const unsigned long long m = __ballot(need);
-            if (m) {
-                while (pool_next >= pool_end && !drained) {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(a.counter + q, (unsigned long long)kPoolChunk);
-                    base = uniform_u64(base);
-                    const int64_t qe = a.total * (q + 1) / a.n_queues;
-                    const int64_t s0 = a.total * q / a.n_queues + (int64_t)base;
-                    if (s0 < qe) {
-                        pool_next = s0;
-                        pool_end = s0 + kPoolChunk < qe ? s0 + kPoolChunk : qe;
-                    } else if (++tried >= a.n_queues) {
-                        drained = true;
-                    } else {
-                        q = q + 1 == a.n_queues ? 0 : q + 1;
-                    }
-                }
-                const int64_t avail = pool_end > pool_next ? pool_end - pool_next : 0;
-                const int rank = __popcll(m & ((1ULL << lane) - 1ULL));
-                if (need) {
-                    if (rank < avail) start(pool_next + rank);
-                    else if (drained) done = true;
-                }
-                const int64_t cnt = __popcll(m);
-                pool_next += cnt < avail ? cnt : avail;
-            }
-        }
-        if (!__any(active)) break;
-        // The body below is one straight-line block for every lane (a lane that is not
-        // stepping computes and discards), so the LUT loads of the new X are issued
-        // together and overlap the next step's normal draw, which does not depend on X.
-        // Inactive lanes hold stale or unset state: they touch no memory except the LUT,
-        // whose coordinates are clamped into the table.
+
+    // One step for every lane, as one straight-line block (a lane that is not stepping
+    // computes and discards), so the LUT loads of the new X are issued together and overlap
+    // the next step's normal draw, which does not depend on X.  Inactive lanes hold stale or
+    // unset state: they touch no memory except the LUT, whose coordinates are clamped into
+    // the table.  A lane whose path stops writes its exit record and goes inactive.
+    // DRAW_FIRST: draw this step's normals at its start instead of overlapping the next
+    // step's draw with the LUT reads (for LDS-resident LUT tiles, whose reads are short).
+    template <bool TILED, bool DRAW_FIRST>
+    __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T) {
+        const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
         const bool stepping = active && !fresh;
+        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
         {
             PathState<R> t = s;
             em_step<R>(t, co, h, sqrth, pn1, pn2);
@@ -257,8 +218,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
         R ex, ey, nx, ny;
         int kind;
         const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        pde_coef<PDE, R>(s.x, s.y, a.pp, co);
-        draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
+        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, co);
+        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
@@ -266,7 +227,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
         const bool capped = s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
             int side;
-            if constexpr (sizeof(R) == 8) side = classify_coord(rectd, a.glob, (double)ex, (double)ey);
+            if constexpr (sizeof(R) == 8) side = classify_coord(rect, a.glob, (double)ex, (double)ey);
             else side = classify_kind(kind, gmask);
             double sc;
             if (side == 4) {
@@ -284,6 +245,161 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
             active = false;
         }
         fresh = false;
+    }
+};
+
+// Wave-private pool of consecutive paths [next, end) taken kPoolChunk at a time from one
+// queue (one atomic per kPoolChunk paths).  With it, the knot of the pool's first path and
+// where that knot ends, found once per chunk with wave-uniform values; a lane whose path
+// lies past that knot searches for its own.
+struct WavePool {
+    int64_t next = 0, end = 0;
+    int ck = 0;
+    int64_t ck_end = 0;
+
+    // a fresh chunk from counter over [qb, qe); false if that queue is drained
+    __device__ __forceinline__ bool take(const PathKernelArgs &a, unsigned long long *counter, int64_t qb,
+                                         int64_t qe, int lane) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(counter, (unsigned long long)kPoolChunk);
+        base = uniform_u64(base);
+        const int64_t s0 = qb + (int64_t)base;
+        if (s0 >= qe) return false;
+        next = s0;
+        end = s0 + kPoolChunk < qe ? s0 + kPoolChunk : qe;
+        ck = find_knot(a.knots, a.n_knots, s0);
+        ck_end = ck + 1 < a.n_knots ? a.knots[ck + 1].path_off : a.total;
+        return true;
+    }
+    __device__ __forceinline__ int knot_of(const PathKernelArgs &a, int64_t gg) const {
+        return gg < ck_end ? ck : find_knot(a.knots, a.n_knots, gg);
+    }
+    // start paths on the lanes that need one; lanes left without a path once `drained`
+    // is set are marked done
+    template <class W>
+    __device__ __forceinline__ void hand_out(const PathKernelArgs &a, W &w, bool need, unsigned long long m,
+                                             bool drained, bool &done, int lane) {
+        const int64_t avail = end > next ? end - next : 0;
+        const int rank = __popcll(m & ((1ULL << lane) - 1ULL));
+        if (need) {
+            if (rank < avail) {
+                const int64_t gg = next + rank;
+                w.start(a, gg, knot_of(a, gg));
+            } else if (drained) {
+                done = true;
+            }
+        }
+        const int64_t cnt = __popcll(m);
+        next += cnt < avail ? cnt : avail;
+    }
+};
+
+// ----------------------------------------------------------------------------
+// Path kernel: one path per lane, persistent (XCD queues) or plain (one launch lane per path)
+// ----------------------------------------------------------------------------
+template <class R, int PDE, bool PERSIST>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
+    const int lane = __lane_id();
+    const LutTile none{nullptr, 0, 0, 0, 0};
+    Walker<R, PDE> w;
+    w.init();
+    bool done = false;
+    if (!PERSIST) {
+        const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (gg < a.total) w.start(a, gg, find_knot(a.knots, a.n_knots, gg));
+        else done = true;
+    }
+    // The wave starts on its XCD's queue, so the paths (and LUT texels) of one XCD's L2
+    // stay in one contiguous range of knots; a drained queue sends the wave on to the next.
+    WavePool pool;
+    bool drained = false;
+    int q = PERSIST ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
+    int tried = 0;
+    while (true) {
+        if (PERSIST) {
+            const bool need = !w.active && !done;
+            const unsigned long long m = __ballot(need);
+            if (m) {
+                while (pool.next >= pool.end && !drained) {
+                    if (pool.take(a, a.counter + q, a.total * q / a.n_queues, a.total * (q + 1) / a.n_queues, lane)) break;
+                    if (++tried >= a.n_queues) drained = true;
+                    else q = q + 1 == a.n_queues ? 0 : q + 1;
+                }
+                pool.hand_out(a, w, need, m, drained, done, lane);
+            }
+        }
+        if (!__any(w.active)) break;
+        w.template step<false, false>(a, none);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// LDS-tiled path kernel (coefficient-LUT problem): one work-group per CU.  The work-group
+// works on one patch at a time: it copies the patch's plane-a tile into LDS, its waves take
+// paths from the patch's queue until it is drained, and once every wave is done it moves to
+// the next patch with paths left.  The 16 taps of a step are then LDS reads; only the
+// x-quads of field 4 (4 per step) and taps outside the tile go to the vector L1.
+// ----------------------------------------------------------------------------
+template <class R> constexpr int tiled_wg() { return 64 * 4 * path_waves<R, PDE_VARCOEF_LUT>(); }
+
+__device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (p[mid].pbeg <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class R>
+__global__ void __launch_bounds__(tiled_wg<R>()) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE_VARCOEF_LUT>())))
+path_kernel_tiled(PathKernelArgs a) {
+    extern __shared__ float4 tile[];
+    __shared__ int s_patch;
+    const int lane = __lane_id();
+    Walker<R, PDE_VARCOEF_LUT> w;
+    w.init();
+    // first patch: the one holding this work-group's share of the batch's path range
+    int p = find_patch(a.patches, a.n_patches, (int64_t)blockIdx.x * a.total / (int64_t)gridDim.x);
+    while (true) {
+        if (threadIdx.x == 0) {
+            int pick = -1;
+            for (int k = 0; k < a.n_patches; ++k) {
+                const int c = p + k < a.n_patches ? p + k : p + k - a.n_patches;
+                const PatchDesc &pd = a.patches[c];
+                const unsigned long long taken = __hip_atomic_load(a.counter + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((int64_t)taken < pd.pend - pd.pbeg) { pick = c; break; }
+            }
+            s_patch = pick;
+        }
+        __syncthreads();
+        p = __builtin_amdgcn_readfirstlane(s_patch);   // uniform: the patch, its tile and queue stay scalar
+        if (p < 0) break;
+        const PatchDesc pd = a.patches[p];
+        const LutTile T{(lds_float4 *)tile, pd.c0, pd.r0, pd.tw, pd.tiled ? pd.th : 0};
+        if (pd.tiled) {
+            const int nt = pd.tw * pd.th;
+            for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+                const int r = i / pd.tw, c = i - r * pd.tw;
+                tile[i] = a.pp.lut.a[(int64_t)(pd.r0 + r) * a.pp.lut.px + (pd.c0 + c)];
+            }
+        }
+        __syncthreads();
+        WavePool pool;
+        bool drained = false, done = false;
+        while (true) {
+            const bool need = !w.active && !done;
+            const unsigned long long m = __ballot(need);
+            if (m) {
+                if (pool.next >= pool.end && !drained)
+                    if (!pool.take(a, a.counter + p, pd.pbeg, pd.pend, lane)) drained = true;
+                pool.hand_out(a, w, need, m, drained, done, lane);
+            }
+            if (!__any(w.active)) break;
+            w.template step<true, true>(a, T);
+        }
+        __syncthreads();   // every wave is done with this tile
     }
 }
 

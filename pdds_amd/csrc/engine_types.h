@@ -74,6 +74,15 @@ struct Records {
     int32_t *steps;
 };
 
+// A patch: knots whose LUT tile (the plane-a texels their stencil rectangle can touch) is the
+// same; their paths are [pbeg, pend) of the batch.  tiled == 0: the tile does not fit in LDS
+// and the patch reads the LUT from global memory.
+struct PatchDesc {
+    int64_t pbeg, pend;
+    int32_t c0, r0, tw, th;
+    int32_t tiled, pad;
+};
+
 struct PathKernelArgs {
     const KnotDesc *knots;
     int32_t n_knots;
@@ -85,6 +94,9 @@ struct PathKernelArgs {
     // dispatched to) and moves to the next queue when its own is drained
     unsigned long long *counter;
     int32_t n_queues;
+    // LDS-tiled launches: one queue per patch (counter[p] counts paths handed out of patch p)
+    const struct PatchDesc *patches;
+    int32_t n_patches;
     Records rec;
     double glob[4];
     double h, sqrth, beta;
@@ -137,6 +149,9 @@ struct PathsArgs {
 };
 
 // Launchers (defined in engine_f64.hip / engine_f32.hip)
+hipError_t launch_path_tiled_f64(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+int path_tiled_wg(int precision);
 hipError_t launch_path_f64(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
 hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
 hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a);

@@ -345,10 +345,27 @@ PDDS_HD const T &lut_at(const void *base, uint32_t byte_off) {
     return *(const T *)((const char *)base + byte_off);
 }
 
+// A rectangle of plane a held in LDS: padded columns [c0, c0 + tw), rows [r0, r0 + th),
+// row-major with row length tw.  th == 0 means no tile.  The pointer is typed in the LDS
+// address space, so tile reads are ds_read_b128 and never merge with the global-memory
+// branch into generic (flat) loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const float4 lds_float4;
+#else
+typedef const float4 lds_float4;   // the host pass never dereferences a tile
+#endif
+struct LutTile {
+    lds_float4 *lds;
+    int c0, r0, tw, th;
+};
+
 // Evaluate fields [0, NF) (NF <= 8) at (x,y):
 //   sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
-template <class R, int NF>
-PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
+// With TILED, a lane whose 4x4 taps lie inside the tile reads fields 0..3 from LDS (one
+// 16-byte LDS read per tap) and the others read them from plane a; the values, and the
+// order of the fused multiply-adds, are the same either way.
+template <class R, int NF, bool TILED>
+PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF]) {
     static_assert(NF <= 8, "at most 8 fields per texel");
     R t = (x - (R)L.ox) * (R)L.sx;
     R v = (y - (R)L.oy) * (R)L.sy;
@@ -358,14 +375,30 @@ PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
     float wx[4], wy[4];
     bspline_wf((float)(t - fi), wx);
     bspline_wf((float)(v - fj), wy);
-    // padded index of tap (a=0, b=0): (j - 1 + kLutPad) * px + (i - 1 + kLutPad), >= 0
-    const uint32_t t0 = (uint32_t)((int)fj + (kLutPad - 1)) * (uint32_t)L.px + (uint32_t)((int)fi + (kLutPad - 1));
+    // padded column / row of tap (a=0, b=0): i - 1 + kLutPad >= 0, j - 1 + kLutPad >= 0
+    const int pc = (int)fi + (kLutPad - 1), pr = (int)fj + (kLutPad - 1);
+    const uint32_t t0 = (uint32_t)pr * (uint32_t)L.px + (uint32_t)pc;
+    bool in = false;
+    lds_float4 *tp = T.lds;
+    if constexpr (TILED) {
+        const int lc = pc - T.c0, lr = pr - T.r0;
+        in = lc >= 0 && lr >= 0 && lc + 3 < T.tw && lr + 3 < T.th;
+        tp = T.lds + (in ? lr * T.tw + lc : 0);
+    }
     float acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t r0 = t0 + (uint32_t)b * (uint32_t)L.px;
+        float4 A[4];
+        if (TILED && in) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) A[a] = tp[b * T.tw + a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) A[a] = lut_at<float4>(L.a, (r0 + (uint32_t)a) * 16u);
+        }
         float row[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) row[f] = 0.0f;
@@ -374,14 +407,13 @@ PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
         const float f4[4] = {quad.x, quad.y, quad.z, quad.w};
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const uint32_t ti = r0 + (uint32_t)a;
             float c[8];
-            const float4 q0 = lut_at<float4>(L.a, ti * 16u);
+            const float4 q0 = A[a];
             c[0] = q0.x; c[1] = q0.y; c[2] = q0.z; c[3] = q0.w;
             if constexpr (NF == 5) {
                 c[4] = f4[a];
             } else if constexpr (NF > 5) {
-                const float4 q1 = lut_at<float4>(L.b, ti * 16u);
+                const float4 q1 = lut_at<float4>(L.b, (r0 + (uint32_t)a) * 16u);
                 c[4] = q1.x; c[5] = q1.y; c[6] = q1.z; c[7] = q1.w;
             }
 #pragma unroll
@@ -392,6 +424,12 @@ PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) out[f] = (R)acc[f];
+}
+
+template <class R, int NF>
+PDDS_HD void lut_eval(const LutView &L, R x, R y, R out[NF]) {
+    const LutTile none{nullptr, 0, 0, 0, 0};
+    lut_eval_t<R, NF, false>(L, none, x, y, out);
 }
 
 }  // namespace pdds

@@ -48,8 +48,10 @@ PDDS_HD void monegros(R x, R y, R &u, R &gx, R &gy, R &lap) {
     lap = dxx + dyy;
 }
 
-template <int PDE, class R>
-PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
+// TILED: the coefficient LUT's fields 0..3 are read from the LDS tile T where it covers X.
+template <int PDE, class R, bool TILED>
+PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, Coef<R> &co) {
+    (void)T;
     if constexpr (PDE == PDE_MONEGROS) {
         R u, gx, gy, lap;
         monegros<R>(x, y, u, gx, gy, lap);
@@ -91,11 +93,17 @@ PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
             co.c = -((R)1.0 + x * x);
         } else {
             R v[5];
-            lut_eval<R, 5>(P.lut, x, y, v);
+            lut_eval_t<R, 5, TILED>(P.lut, T, x, y, v);
             co.s00 = v[0]; co.s01 = 0; co.s10 = 0; co.s11 = v[1];
             co.b0 = v[2]; co.b1 = v[3]; co.c = v[4];
         }
     }
+}
+
+template <int PDE, class R>
+PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
+    const LutTile none{nullptr, 0, 0, 0, 0};
+    pde_coef_t<PDE, R, false>(x, y, P, none, co);
 }
 
 // Dirichlet data g = u (all shipped problems are manufactured solutions)

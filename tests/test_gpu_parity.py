@@ -119,10 +119,11 @@ def test_solve_knot_bit_exact(eng, knot, quirk):
     assert got["counters"]["exits"] == [int(v) for v in ref["cnt"]]
 
 
-@pytest.mark.parametrize("plain", [0, 1], ids=["persistent", "plain"])
-def test_solve_knot_lut_bit_exact(eng, plain):
-    """The assembly path kernel on the coefficient-LUT problem (the benchmark's kernel), persistent
-    queue and plain launch: engine == oracle bit for bit."""
+@pytest.mark.parametrize("plain,lut_lds", [(0, 1), (0, 0), (1, 0)], ids=["tiled", "persistent", "plain"])
+def test_solve_knot_lut_bit_exact(eng, plain, lut_lds):
+    """The assembly path kernels on the coefficient-LUT problem (the benchmark's kernels): the
+    LDS-tiled kernel, the persistent kernel reading the LUT from global memory, and the plain
+    launch: engine == oracle bit for bit."""
     from pdds_amd.engine import bvp, default_run
     from pdds_amd.workloads import relative_fac
     lut, olut = _lut_pair(eng)
@@ -130,7 +131,7 @@ def test_solve_knot_lut_bit_exact(eng, plain):
     m = _product_mesh((-1.0, -1.0), (1.0, 1.0), 4, 8, fac)
     knot, n = 43, 3000                  # four non-empty stencil sides
     st = m.stencil(knot)
-    run = default_run(h=1e-3, seed=5, sub_batch=32, item_paths=640, plain=plain)
+    run = default_run(h=1e-3, seed=5, sub_batch=32, item_paths=640, plain=plain, lut_lds=lut_lds)
     got = eng.solve_knot(m, bvp("varcoef_lut", lut=lut), run, knot, n)
     pb = O.problem(O.PDE_VARCOEF_LUT, lut=olut)
     paths = O.simulate(pb, O.run_params(1e-3, 5, threads=8), m.x0[knot], m.y0[knot], st["rect"], knot, 0, n)
@@ -139,6 +140,28 @@ def test_solve_knot_lut_bit_exact(eng, plain):
     assert np.array_equal(got["Gvar"], ref["Gvar"])
     assert got["B"] == ref["B"] and got["BB"] == ref["BB"]
     assert got["counters"]["path_steps"] == int(paths["steps"].sum())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_assemble_lut_tiled_equals_global(eng, precision):
+    """A whole mesh on the coefficient-LUT problem: the LDS-tiled kernel (knots regrouped by
+    tile, one queue per tile) gives exactly the G, G variances, B and BB of the kernel that reads
+    the LUT from global memory, knot by knot."""
+    from pdds_amd.engine import bvp, default_run
+    from pdds_amd.workloads import relative_fac
+    # 256^2 LUT on 4x4 subdomains: the interface-segment tiles (132 x 68 texels) fit in LDS,
+    # the crossing-knot tiles (132 x 132) do not, so both patch kinds run in one launch
+    lut, _ = _lut_pair(eng, n=256)
+    m = _product_mesh((-1.0, -1.0), (1.0, 1.0), 4, 8, relative_fac((-1.0, -1.0), (1.0, 1.0), 4, 8, 1.0))
+    outs = []
+    for lds in (1, 0):
+        run = default_run(h=2e-3, seed=21, precision=precision, lut_lds=lds)
+        outs.append(eng.assemble(m, bvp("varcoef_lut", lut=lut), run, 1500))
+    a, b = outs
+    for k in ("G", "Gvar", "B", "BB"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["counters"]["path_steps"] == b["counters"]["path_steps"]
+    assert a["counters"]["exits"] == b["counters"]["exits"]
 
 
 def test_assemble_fixture_statistical_parity(eng, golden):

@@ -170,6 +170,8 @@ struct AsmKnot {
     int stencil;
     int64_t n;
     int64_t out_off;
+    bool has_k2 = false;    // per-knot Helmholtz parameter (serving), else the problem's
+    double k2 = 0.0;
 };
 
 struct AsmOut {
@@ -293,6 +295,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         int64_t tot = 0;
         while (k1 < knots.size() && (k1 == k0 || tot + knots[ord[k1]].n <= budget)) tot += knots[ord[k1++]].n;
         const int nk = (int)(k1 - k0);
+        if (tot >= ((int64_t)1 << 32)) return fail(e, PDDS_ERR_UNSUPPORTED, "a knot with 2^32 or more paths");
         std::vector<KnotDesc> kd(nk);
         std::vector<ItemDesc> items;
         int64_t part = 0, off = 0;
@@ -306,6 +309,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             nnz_cap = std::max(nnz_cap, sd.csr_nnz);
             kd[q].x0 = k.x0; kd[q].y0 = k.y0; kd[q].knot_id = k.id; kd[q].stencil = k.stencil;
             kd[q].n_paths = k.n; kd[q].path_off = off; kd[q].out_off = k.out_off;
+            kd[q].k2 = k.has_k2 ? k.k2 : pp.k2;
             kd[q].item_begin = (int64_t)items.size();
             for (int64_t p0 = 0; p0 < k.n; p0 += P) {
                 ItemDesc it;

@@ -66,7 +66,7 @@ PDDS_HD int classify_kind(int kind, int gmask) {
 // Gaussian RBF exp(-|p - X|^2 / c^2) (BVPs/BVP.hpp:84-87) with 1/c^2 precomputed
 PDDS_HD double rbf(double px, double py, double x, double y, double inv_c2) {
     double dx = px - x, dy = py - y;
-    return dexp(-(dx * dx + dy * dy) * inv_c2);
+    return dexp_nonpos(-(dx * dx + dy * dy) * inv_c2);
 }
 
 // ----------------------------------------------------------------------------
@@ -76,7 +76,7 @@ template <class R> struct PathState {
     R x, y, Y, Z, xi;
     R zc, xc;          // Kahan compensations (FP32 only)
     R pn1, pn2;        // pending normals of the odd step (FP32 only)
-    int64_t st;
+    uint32_t st;       // steps taken (below 2^31: the phase-1 cap is below 2^30)
     uint32_t path;     // path index within its knot
 };
 
@@ -104,36 +104,43 @@ __device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h,
     s.y = ny;
 }
 
-// Normal pair of step s.st for this lane's path.
+// Normal pair of step s.st for this lane's path.  FP32: one Philox call per even step gives
+// the normals of steps st and st + 1; `even` is the parity of st, which the path kernels keep
+// the same for every lane of a wave (paths only start on odd wave iterations), so the draw is a
+// wave-uniform branch taken every other step.
 template <class R>
 __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uint32_t stream, uint32_t knot,
-                                             R &n1, R &n2) {
+                                             R &n1, R &n2, bool even) {
     if constexpr (sizeof(R) == 8) {
+        (void)even;
         double a, b;
         normals_f64(seed, stream, knot, s.path, s.st, a, b);
         n1 = a; n2 = b;
     } else {
-        // one Philox call per even step gives the normals of steps st and st + 1; computed
-        // for every lane and selected (lanes of one wave rarely share the step parity, so a
-        // branch would run both sides anyway, and it would split the straight-line step)
-        const bool even = (s.st & 1) == 0;
-        const int64_t call = s.st >> 1;
-        Philox4 w = philox4x32_10((uint32_t)call, s.path, knot, (uint32_t)(call >> 32), seed, stream);
-        const float sc = 5.9604644775390625e-08f;
-        float u1 = ((float)(w.v[0] >> 8) + 0.5f) * sc;
-        float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
-        float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
-        float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
-        float r1 = __builtin_sqrtf(-2.0f * RM<float>::log(u1));
-        float r2 = __builtin_sqrtf(-2.0f * RM<float>::log(u3));
-        const float a1 = r1 * __builtin_amdgcn_cosf(u2), a2 = r1 * __builtin_amdgcn_sinf(u2);
-        const float b1 = r2 * __builtin_amdgcn_cosf(u4), b2 = r2 * __builtin_amdgcn_sinf(u4);
-        n1 = even ? a1 : s.pn1;
-        n2 = even ? a2 : s.pn2;
-        s.pn1 = even ? b1 : s.pn1;
-        s.pn2 = even ? b2 : s.pn2;
+        if (even) {
+            const uint32_t call = s.st >> 1;
+            Philox4 w = philox4x32_10(call, s.path, knot, 0u, seed, stream);
+            const float sc = 5.9604644775390625e-08f;
+            float u1 = ((float)(w.v[0] >> 8) + 0.5f) * sc;
+            float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
+            float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
+            float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
+            float r1 = __builtin_sqrtf(-2.0f * RM<float>::log(u1));
+            float r2 = __builtin_sqrtf(-2.0f * RM<float>::log(u3));
+            n1 = r1 * __builtin_amdgcn_cosf(u2);
+            n2 = r1 * __builtin_amdgcn_sinf(u2);
+            s.pn1 = r2 * __builtin_amdgcn_cosf(u4);
+            s.pn2 = r2 * __builtin_amdgcn_sinf(u4);
+        } else {
+            n1 = s.pn1;
+            n2 = s.pn2;
+        }
     }
 }
+
+// Wave iterations on which a lane may start a path: FP32 paths start only on odd iterations,
+// so that a stepping lane's step count has the parity of the iteration; FP64 paths any time.
+template <class R> __device__ __forceinline__ bool may_start(int it) { return sizeof(R) == 8 || (it & 1); }
 
 // Lane 0's value in every lane, held in scalar registers: the queue bookkeeping of a wave
 // (pool bounds, queue index) is uniform and does not need vector registers.
@@ -165,11 +172,12 @@ struct Walker {
     R rect[4];             // stencil rectangle in path precision (FP64 classifies on it)
     uint32_t kid;
     int gmask;
-    int64_t g;             // batch path index (record slot)
+    uint32_t g;            // batch path index (record slot; a batch holds fewer than 2^32 paths)
+    R k2;                  // Helmholtz parameter of the path's knot
     bool active, fresh;
 
     __device__ __forceinline__ void init() {
-        pn1 = 0; pn2 = 0; kid = 0; gmask = 0; g = -1;
+        pn1 = 0; pn2 = 0; kid = 0; gmask = 0; g = 0; k2 = 0;
         active = false; fresh = false;
     }
 
@@ -180,7 +188,8 @@ struct Walker {
         for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
         gmask = sd.gmask;
         kid = kd.knot_id;
-        g = gg;
+        k2 = (R)kd.k2;
+        g = (uint32_t)gg;
         s.path = (uint32_t)(gg - kd.path_off);
         s.x = (R)kd.x0; s.y = (R)kd.y0;
         s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
@@ -196,11 +205,12 @@ struct Walker {
     // the table.  A lane whose path stops writes its exit record and goes inactive.
     // DRAW_FIRST: draw this step's normals at its start instead of overlapping the next
     // step's draw with the LUT reads (for LDS-resident LUT tiles, whose reads are short).
+    // it: the wave's iteration count (stepping lanes have s.st of its parity)
     template <bool TILED, bool DRAW_FIRST>
-    __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T) {
+    __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T, int it) {
         const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
         const bool stepping = active && !fresh;
-        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
+        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2, (it & 1) == 0);
         {
             PathState<R> t = s;
             em_step<R>(t, co, h, sqrth, pn1, pn2);
@@ -218,13 +228,13 @@ struct Walker {
         R ex, ey, nx, ny;
         int kind;
         const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, co);
-        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2);
+        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, k2, co);
+        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
         const bool go = d < beta * nrm * sqrth;
-        const bool capped = s.st >= a.max_steps;
+        const bool capped = (int64_t)s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
             int side;
             if constexpr (sizeof(R) == 8) side = classify_coord(rect, a.glob, (double)ex, (double)ey);
@@ -315,8 +325,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
     bool drained = false;
     int q = PERSIST ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
     int tried = 0;
-    while (true) {
-        if (PERSIST) {
+    for (int it = 1;; ++it) {
+        const bool can_start = may_start<R>(it);
+        if (PERSIST && can_start) {
             const bool need = !w.active && !done;
             const unsigned long long m = __ballot(need);
             if (m) {
@@ -328,8 +339,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
                 pool.hand_out(a, w, need, m, drained, done, lane);
             }
         }
-        if (!__any(w.active)) break;
-        w.template step<false, false>(a, none);
+        if (!__any(w.active)) {
+            if (can_start || !PERSIST) break;
+            continue;   // idle until the next iteration on which paths may start
+        }
+        w.template step<false, false>(a, none, it);
     }
 }
 
@@ -388,16 +402,22 @@ path_kernel_tiled(PathKernelArgs a) {
         __syncthreads();
         WavePool pool;
         bool drained = false, done = false;
-        while (true) {
-            const bool need = !w.active && !done;
-            const unsigned long long m = __ballot(need);
-            if (m) {
-                if (pool.next >= pool.end && !drained)
-                    if (!pool.take(a, a.counter + p, pd.pbeg, pd.pend, lane)) drained = true;
-                pool.hand_out(a, w, need, m, drained, done, lane);
+        for (int it = 1;; ++it) {
+            const bool can_start = may_start<R>(it);
+            if (can_start) {
+                const bool need = !w.active && !done;
+                const unsigned long long m = __ballot(need);
+                if (m) {
+                    if (pool.next >= pool.end && !drained)
+                        if (!pool.take(a, a.counter + p, pd.pbeg, pd.pend, lane)) drained = true;
+                    pool.hand_out(a, w, need, m, drained, done, lane);
+                }
             }
-            if (!__any(w.active)) break;
-            w.template step<true, true>(a, T);
+            if (!__any(w.active)) {
+                if (can_start) break;
+                continue;
+            }
+            w.template step<true, true>(a, T, it);
         }
         __syncthreads();   // every wave is done with this tile
     }
@@ -430,12 +450,12 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
     R ex = s.x, ey = s.y;
     for (int phase = 0; phase < (a.phase2 ? 2 : 1); ++phase) {
         const R beta = phase == 0 ? (R)a.beta : (R)0;
-        int64_t st0 = s.st;
+        const uint32_t st0 = s.st;
         bool go = true;
         while (go) {
             R n1, n2, nx, ny;
             int kind;
-            draw_normals<R>(s, a.seed, a.stream, a.knot, n1, n2);
+            draw_normals<R>(s, a.seed, a.stream, a.knot, n1, n2, (s.st & 1) == 0);
             em_step<R>(s, co, h, sqrth, n1, n2);
             s.st += 1;
             R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
@@ -443,7 +463,7 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
             R ga = nx * co.s00 + ny * co.s10;
             R gb = nx * co.s01 + ny * co.s11;
             R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-            go = (d < beta * nrm * sqrth) && (s.st - st0 < a.max_steps);
+            go = (d < beta * nrm * sqrth) && ((int64_t)(s.st - st0) < a.max_steps);
         }
         double *o[5];
         int64_t *os;
@@ -454,7 +474,7 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
         if (o[2]) o[2][i] = (double)s.Y;
         if (o[3]) o[3][i] = (double)s.Z;
         if (o[4]) o[4][i] = (double)s.xi;
-        if (os) os[i] = s.st - st0;
+        if (os) os[i] = (int64_t)(s.st - st0);
     }
 }
 

@@ -30,6 +30,7 @@ struct KnotDesc {
     int64_t out_off;        // first value of the knot's G row in the CSR value array
     int64_t item_begin;     // first work item of the knot
     int64_t item_count;
+    double k2;              // Helmholtz parameter of the knot's problem
 };
 
 // One distinct stencil: stopping rectangle, four side lists (N,S,E,W), Psi^-1 per side,

@@ -66,6 +66,56 @@ PDDS_HD double dexp(double x) {
     return (p * pow2i(ki + 600)) * pow2i(-600);
 }
 
+// fma(p, r, c) with the constant c taken from a scalar register pair: one v_fma_f64 per Horner
+// step.  (Left to itself the compiler keeps the coefficients of a loop-resident polynomial in
+// vector registers and evaluates each step as a copy plus a two-address v_fmac_f64.)
+PDDS_HD double fma_k(double p, double r, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(p), "v"(r), "s"(c));
+    return d;
+#else
+    return __builtin_fma(p, r, c);
+#endif
+}
+
+// exp_poly with one instruction per Horner step (same coefficients, same fused operations)
+PDDS_HD double exp_poly_k(double r) {
+    double p = 1.1470745597729725e-11;
+    p = fma_k(p, r, 1.6059043836821613e-10);
+    p = fma_k(p, r, 2.08767569878681e-09);
+    p = fma_k(p, r, 2.505210838544172e-08);
+    p = fma_k(p, r, 2.755731922398589e-07);
+    p = fma_k(p, r, 2.7557319223985893e-06);
+    p = fma_k(p, r, 2.48015873015873e-05);
+    p = fma_k(p, r, 0.0001984126984126984);
+    p = fma_k(p, r, 0.001388888888888889);
+    p = fma_k(p, r, 0.008333333333333333);
+    p = fma_k(p, r, 0.041666666666666664);
+    p = fma_k(p, r, 0.16666666666666666);
+    p = fma_k(p, r, 0.5);
+    p = fma_k(p, r, 1.0);
+    p = fma_k(p, r, 1.0);
+    return p;
+}
+
+// dexp(x) for x <= 0, bit for bit, without branches: the value of dexp's own steps, with the
+// underflow (x < -745.2 -> 0) and subnormal scalings as selects.  The RBF moments of the
+// reduction evaluate one of these per (exit, stencil entry).
+PDDS_HD double dexp_nonpos(double x) {
+    const bool under = x < -745.2;
+    const double xc = under ? 0.0 : x;
+    const double k = __builtin_floor(xc * kLog2e + 0.5);
+    double r = __builtin_fma(-k, kLn2Hi, xc);
+    r = __builtin_fma(-k, kLn2Lo, r);
+    const double p = exp_poly_k(r);
+    const int ki = (int)k;                       // in [-1075, 0]
+    const bool sub = ki < -1022;
+    const double v = p * pow2i(sub ? ki + 600 : ki);
+    const double w = sub ? v * pow2i(-600) : v;
+    return under ? 0.0 : w;
+}
+
 PDDS_HD double dexpm1(double x) {
     if (__builtin_fabs(x) <= 0.5) {
         double p = 2.8114572543455206e-15;

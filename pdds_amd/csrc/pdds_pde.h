@@ -20,7 +20,7 @@ template <class R> struct Coef {
 };
 
 struct PdeParams {
-    double k2;             // HELMHOLTZ
+    double k2;             // HELMHOLTZ (launches without per-knot values)
     LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c
 };
 
@@ -49,8 +49,10 @@ PDDS_HD void monegros(R x, R y, R &u, R &gx, R &gy, R &lap) {
 }
 
 // TILED: the coefficient LUT's fields 0..3 are read from the LDS tile T where it covers X.
+// k2: the Helmholtz parameter of this path's problem (per knot, so one launch can carry
+// requests with different k^2).
 template <int PDE, class R, bool TILED>
-PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, Coef<R> &co) {
+PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Coef<R> &co) {
     (void)T;
     if constexpr (PDE == PDE_MONEGROS) {
         R u, gx, gy, lap;
@@ -71,7 +73,7 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, Coef<R> 
             co.c = 0; co.f = flap;
         } else {
             R uu = sx * sy + x * x + y * y;
-            co.c = -(R)P.k2; co.f = flap + (R)P.k2 * uu;
+            co.c = -k2; co.f = flap + k2 * uu;
         }
     } else {  // VARCOEF, VARCOEF_LUT
         R sxn, cxn, syn, cyn;
@@ -103,7 +105,7 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, Coef<R> 
 template <int PDE, class R>
 PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
     const LutTile none{nullptr, 0, 0, 0, 0};
-    pde_coef_t<PDE, R, false>(x, y, P, none, co);
+    pde_coef_t<PDE, R, false>(x, y, P, none, (R)P.k2, co);
 }
 
 // Dirichlet data g = u (all shipped problems are manufactured solutions)

@@ -166,6 +166,68 @@ pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_ptr, const 
                        const double *G, const double *B, double *u, double tol, int32_t maxit,
                        int32_t *iters, double *relres);
 
+/* ---- serving (no counterpart in the reference): a continuous-batching request loop ----
+   Requests are point evaluations of u or full interface solves on the rectangle-mesh
+   geometry.  A host pool prepares each request (a solve builds its mesh and Psi^-1 tables);
+   one device thread then runs rounds: each round admits prepared requests in arrival order
+   while their device footprint fits hbm_budget and fewer than max_inflight are admitted,
+   packs the not-yet-run work of every admitted request (point paths first, then solve knots,
+   at most round_paths paths) into one persistent launch, and completes every request whose
+   work is done (a solve after its device BiCGSTAB).  A solve larger than a round is spread
+   over rounds, so requests arriving meanwhile join the next round.  The paths of a request
+   use its own Philox stream, and each of its sums has a fixed order: its result is the same,
+   bit for bit, alone or in any mix.  The server uses the engine's stream and lock; other
+   engine calls made meanwhile wait for the current round. */
+typedef struct pdds_server pdds_server;
+typedef struct {
+    pdds_problem problem;   /* PDE of every request (the Helmholtz k2 comes with each request) */
+    pdds_run run;           /* h, seed, precision and launch parameters; stream is per request */
+    double domain[4];       /* x0, y0, x1, y1 of every request's problem */
+    int64_t hbm_budget;     /* device bytes for admitted requests' tables and records (0: 16 GiB) */
+    int32_t max_inflight;   /* requests admitted at once (0: 256) */
+    int32_t prep_threads;   /* host threads preparing requests (0: 4) */
+    int64_t round_paths;    /* paths per round (0: 2^25) */
+} pdds_server_config;
+
+#define PDDS_REQ_POINT 0
+#define PDDS_REQ_SOLVE 1
+typedef struct {
+    int32_t kind;           /* PDDS_REQ_POINT or PDDS_REQ_SOLVE */
+    uint32_t stream;        /* Philox key word 1 of this request's paths */
+    double k2;              /* Helmholtz parameter (PDDS_PDE_HELMHOLTZ) */
+    int64_t n_paths;        /* paths (point) or paths per Monte Carlo knot (solve) */
+    double x, y;            /* point */
+    int32_t n_sub, n_node;  /* solve: subdomains per side, knots per interface */
+    double fac[4];          /* solve: RBF shape per stencil kind */
+    double tol;             /* solve: BiCGSTAB relative tolerance (0: 1e-10) */
+} pdds_request;
+
+typedef struct {
+    int32_t kind;
+    pdds_status status;     /* PDDS_OK, or why the request failed (a singular mesh, ...) */
+    int32_t n_knots;        /* solve: unknowns (u holds min(n_knots, cap) of them, knot order) */
+    int32_t iters;          /* solve: BiCGSTAB iterations */
+    double relres;          /* solve: final relative residual */
+    double u, u_var;        /* point: estimate and per-path variance */
+    int64_t path_steps;
+    double queue_ms;        /* submit -> admitted to a round */
+    double latency_ms;      /* submit -> done */
+} pdds_result;
+
+pdds_status pdds_server_create(pdds_engine *e, const pdds_server_config *cfg, pdds_server **out);
+/* finishes the requests already submitted, then stops */
+void pdds_server_destroy(pdds_server *s);
+/* PDDS_ERR_FULL if the request alone exceeds the HBM budget */
+pdds_status pdds_server_submit(pdds_server *s, const pdds_request *r, uint64_t *ticket);
+/* PDDS_ERR_NOT_READY while the request runs; once done fills *res (and u for a solve) and
+   releases the ticket */
+pdds_status pdds_server_poll(pdds_server *s, uint64_t ticket, pdds_result *res, double *u, int64_t cap);
+/* as poll, waiting for the request */
+pdds_status pdds_server_wait(pdds_server *s, uint64_t ticket, pdds_result *res, double *u, int64_t cap);
+/* totals since creation: requests submitted and done, launch rounds, path-steps, device busy ms */
+pdds_status pdds_server_stats(pdds_server *s, int64_t *submitted, int64_t *done, int64_t *rounds,
+                              int64_t *path_steps, double *busy_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,29 @@ class Counters(C.Structure):
                 "wg_size": self.wg_size, "item_paths": self.item_paths, "sub_batch": self.sub_batch}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("problem", Problem), ("run", Run), ("domain", C.c_double * 4), ("hbm_budget", C.c_int64),
+                ("max_inflight", C.c_int32), ("prep_threads", C.c_int32), ("round_paths", C.c_int64)]
+
+
+REQ_POINT, REQ_SOLVE = 0, 1
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("stream", C.c_uint32), ("k2", C.c_double), ("n_paths", C.c_int64),
+                ("x", C.c_double), ("y", C.c_double), ("n_sub", C.c_int32), ("n_node", C.c_int32),
+                ("fac", C.c_double * 4), ("tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("status", C.c_int32), ("n_knots", C.c_int32), ("iters", C.c_int32),
+                ("relres", C.c_double), ("u", C.c_double), ("u_var", C.c_double), ("path_steps", C.c_int64),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = {
     "pdds_engine_create": (C.c_int32, [C.c_int, C.POINTER(C.c_void_p)]),
     "pdds_engine_destroy": (None, [C.c_void_p]),
@@ -75,6 +98,12 @@ EXPORTS = {
     "pdds_point_eval": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _d, C.c_int32, _d, _d, _i64,
                                     _d, _d, C.POINTER(Counters)]),
     "pdds_solve": (C.c_int32, [C.c_void_p, C.c_int32, _i64, _i32, _d, _d, _d, C.c_double, C.c_int32, _i32, _d]),
+    "pdds_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "pdds_server_destroy": (None, [C.c_void_p]),
+    "pdds_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "pdds_server_poll": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(Result), _d, C.c_int64]),
+    "pdds_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(Result), _d, C.c_int64]),
+    "pdds_server_stats": (C.c_int32, [C.c_void_p, _i64, _i64, _i64, _i64, _d]),
 }
 
 _LIB = None

@@ -15,94 +15,21 @@
 #include "engine_types.h"
 #include "mesh.h"
 #include "solver.h"
+#include "engine_internal.h"
 
 using namespace pdds;
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
-}  // namespace
-
-// Stencil tables of one mesh in device memory (kept resident across calls).
-struct DevTables {
-    uint64_t mesh_id = 0;
-    int quirk = -1;
-    int n_sten = 0;
-    StencilDesc *sten = nullptr;
-    double *entx = nullptr, *enty = nullptr, *ipsi = nullptr;
-    int8_t *ents = nullptr;
-    int32_t *e1 = nullptr, *e2 = nullptr;
-    std::vector<StencilDesc> host_sten;   // sizes for launch planning
-    size_t bytes = 0;
-    void release() {
-        void *ps[] = {sten, entx, enty, ipsi, ents, e1, e2};
-        for (void *p : ps)
-            if (p) (void)hipFree(p);
-        sten = nullptr; entx = enty = ipsi = nullptr; ents = nullptr; e1 = e2 = nullptr;
-        mesh_id = 0; quirk = -1; n_sten = 0; host_sten.clear(); bytes = 0;
-    }
-};
-
-struct pdds_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-    std::mutex mu;
-    int n_cu = 0;
-    std::string name;
-    std::vector<DevBuf> bufs;   // grow-only scratch, indexed by slot
-    DevTables tables[2];        // two most recent (mesh, quirk) table sets
-    int tables_next = 0;
-    int64_t record_budget = (int64_t)24 << 30;   // bytes of exit records per assembly batch
-    int path_wgs_per_cu = 8;                      // persistent path-kernel work-groups per CU
-    size_t lds_max = 64 * 1024;                   // LDS bytes a work-group may use
-};
-
-struct pdds_lut {
-    pdds_engine *e = nullptr;
-    float *d = nullptr;
-    LutView view{};
-    int nx = 0, ny = 0, nf = 0, mode = 0;
-};
-
-static uint64_t next_mesh_id() {
+uint64_t next_mesh_id() {
     static std::mutex mu;
     static uint64_t id = 0;
     std::lock_guard<std::mutex> lk(mu);
     return ++id;
 }
 
-struct pdds_mesh {
-    Mesh m;
-    uint64_t id;
-    pdds_mesh(double a, double b, double c, double d, int n, int k, const double f[4])
-        : m(a, b, c, d, n, k, f), id(next_mesh_id()) {}
-};
-
-namespace {
-
-enum BufSlot {
-    B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_RECX, B_RECY,
-    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_PATCH, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
-    B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
-};
-
 pdds_status fail(pdds_engine *e, pdds_status s, const std::string &msg) {
     if (e) e->err = msg;
     return s;
 }
-
-#define HIPCHK(expr)                                                                               \
-    do {                                                                                           \
-        hipError_t _r = (expr);                                                                    \
-        if (_r != hipSuccess)                                                                      \
-            return fail(e, PDDS_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_r));      \
-    } while (0)
 
 // grow-only device buffer for slot i
 pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
@@ -116,16 +43,6 @@ pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
         b.n = bytes;
     }
     *out = b.p;
-    return PDDS_OK;
-}
-
-template <class T>
-pdds_status upload(pdds_engine *e, int slot, const std::vector<T> &v, T **out) {
-    void *p;
-    pdds_status s = dbuf(e, slot, v.size() * sizeof(T), &p);
-    if (s) return s;
-    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, e->stream));
-    *out = (T *)p;
     return PDDS_OK;
 }
 
@@ -162,24 +79,6 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
         return fail(e, PDDS_ERR_INVALID, "negative launch parameter");
     return PDDS_OK;
 }
-
-// One generic assembly over a list of knots sharing the stencil table.
-struct AsmKnot {
-    double x0, y0;
-    uint32_t id;
-    int stencil;
-    int64_t n;
-    int64_t out_off;
-    bool has_k2 = false;    // per-knot Helmholtz parameter (serving), else the problem's
-    double k2 = 0.0;
-};
-
-struct AsmOut {
-    std::vector<KnotStat> kstats;
-    double kernel_ms = 0, reduce_ms = 0;
-    int64_t items = 0;
-    int grid = 0, wg = 0, P = 0, S = 0;
-};
 
 // Paths per work item of the reduction: a multiple of S, about N/16, at most 8192.
 int choose_P(int64_t n, int S) {
@@ -238,9 +137,9 @@ void lut_tile_of(const LutView &L, const double rect[4], int &c0, int &r0, int &
 }
 
 pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
-                         const DevTables &T, const std::vector<AsmKnot> &knots, int64_t n_vals,
-                         double *G_host, double *Gvar_host, AsmOut &out) {
-    const std::vector<StencilDesc> &sten = T.host_sten;
+                         const std::vector<const DevTables *> &tabs, const std::vector<AsmKnot> &knots,
+                         int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out) {
+    auto host_sd = [&](const AsmKnot &k) -> const StencilDesc & { return tabs[k.table]->host_sten[k.stencil]; };
     PdeParams pp;
     pdds_status st = make_params(e, prob, pp);
     if (st) return st;
@@ -267,7 +166,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         std::vector<int> rank(knots.size());
         for (size_t i = 0; i < knots.size(); ++i) {
             int c0, r0, tw, th;
-            lut_tile_of(pp.lut, sten[knots[i].stencil].rect, c0, r0, tw, th);
+            lut_tile_of(pp.lut, host_sd(knots[i]).rect, c0, r0, tw, th);
             const int fits = (size_t)tw * th * sizeof(float4) <= tile_cap ? 1 : 0;
             tkey[i] = {c0, r0, tw, th, fits};
             const std::array<int, 4> k4 = fits ? std::array<int, 4>{c0, r0, tw, th} : std::array<int, 4>{-1, -1, -1, -1};
@@ -285,6 +184,12 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     double *dG, *dGv;
     if ((st = dbuf(e, B_G, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dG = (double *)p;
     if ((st = dbuf(e, B_GVAR, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dGv = (double *)p;
+    std::vector<TableDesc> tdesc(tabs.size());
+    for (size_t t = 0; t < tabs.size(); ++t)
+        tdesc[t] = TableDesc{tabs[t]->sten, tabs[t]->entx, tabs[t]->enty, tabs[t]->ents, tabs[t]->e1, tabs[t]->e2,
+                             tabs[t]->ipsi};
+    TableDesc *dtab;
+    if ((st = upload(e, B_TABLES, tdesc, &dtab))) return st;
     double kernel_ms = 0.0, reduce_ms = 0.0;
     int grid_used = 0, Pmax_all = 0;
     int64_t items_all = 0;
@@ -302,12 +207,13 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         int ne_cap = 1, nnz_cap = 1;
         for (int q = 0; q < nk; ++q) {
             const AsmKnot &k = knots[ord[k0 + q]];
-            const StencilDesc &sd = sten[k.stencil];
+            const StencilDesc &sd = host_sd(k);
             const int P = r->item_paths > 0 ? r->item_paths : choose_P(k.n, S);
             Pmax_all = std::max(Pmax_all, P);
             ne_cap = std::max(ne_cap, sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3]);
             nnz_cap = std::max(nnz_cap, sd.csr_nnz);
             kd[q].x0 = k.x0; kd[q].y0 = k.y0; kd[q].knot_id = k.id; kd[q].stencil = k.stencil;
+            kd[q].table = k.table; kd[q].stream = k.has_stream ? k.stream : r->stream;
             kd[q].n_paths = k.n; kd[q].path_off = off; kd[q].out_off = k.out_off;
             kd[q].k2 = k.has_k2 ? k.k2 : pp.k2;
             kd[q].item_begin = (int64_t)items.size();
@@ -375,7 +281,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         // path kernel
         PathKernelArgs pa;
         std::memset(&pa, 0, sizeof(pa));
-        pa.knots = dk; pa.n_knots = nk; pa.sten = T.sten; pa.total = tot; pa.base = 0; pa.counter = dctr; pa.rec = rec;
+        pa.knots = dk; pa.n_knots = nk; pa.tables = dtab; pa.total = tot; pa.base = 0; pa.counter = dctr; pa.rec = rec;
         for (int i = 0; i < 4; ++i) pa.glob[i] = glob[i];
         pa.h = r->h; pa.sqrth = std::sqrt(r->h); pa.beta = r->beta;
         pa.seed = r->seed; pa.stream = r->stream; pa.max_steps = max_steps; pa.pp = pp;
@@ -413,8 +319,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         // reduction + finalize
         ReduceArgs ra;
         std::memset(&ra, 0, sizeof(ra));
-        ra.knots = dk; ra.sten = T.sten; ra.ent_x = T.entx; ra.ent_y = T.enty; ra.ent_side = T.ents;
-        ra.csr_e1 = T.e1; ra.csr_e2 = T.e2; ra.ipsi = T.ipsi; ra.items = dit; ra.rec = rec;
+        ra.knots = dk; ra.tables = dtab; ra.items = dit; ra.rec = rec;
         ra.stats = dstats; ra.partials = dpart; ra.ne_cap = ne_cap; ra.nnz_cap = nnz_cap; ra.S = S;
         const int64_t nit = (int64_t)items.size();
         for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
@@ -423,7 +328,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("reduce launch: ") + hipGetErrorString(lr));
         }
         FinalizeArgs f;
-        f.knots = dk; f.sten = T.sten; f.items = dit; f.stats = dstats; f.partials = dpart;
+        f.knots = dk; f.tables = dtab; f.items = dit; f.stats = dstats; f.partials = dpart;
         f.n_knots = nk; f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
         hipError_t lr = launch_finalize(nk, 256, e->stream, f);
         if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
@@ -522,8 +427,6 @@ void fill_counters(pdds_counters *ctr, const AsmOut &o, double total_ms) {
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -815,7 +718,7 @@ pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         kmap.push_back(q);
     }
     AsmOut o;
-    st = run_assembly(e, p, r, M.glob, *T, knots, M.row_ptr.back(), G, G_var, o);
+    st = run_assembly(e, p, r, M.glob, {T}, knots, M.row_ptr.back(), G, G_var, o);
     if (st) return st;
     for (size_t i = 0; i < knots.size(); ++i) { B[kmap[i]] = o.kstats[i].B; BB[kmap[i]] = o.kstats[i].BB; }
     fill_counters(ctr, o, now_ms() - t0);
@@ -847,7 +750,7 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
     if ((st = mesh_dev_tables(e, m, r->quirk_last_side, &T))) return st;
     std::vector<AsmKnot> knots = {AsmKnot{M.x0[knot], M.y0[knot], (uint32_t)knot, M.stencil_of[knot], n_paths, 0}};
     AsmOut o;
-    st = run_assembly(e, p, r, M.glob, *T, knots, (int64_t)M.stencils[M.stencil_of[knot]].cols.size(), G, G_var, o);
+    st = run_assembly(e, p, r, M.glob, {T}, knots, (int64_t)M.stencils[M.stencil_of[knot]].cols.size(), G, G_var, o);
     if (st) return st;
     *B = o.kstats[0].B;
     *BB = o.kstats[0].BB;
@@ -896,7 +799,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
         kmap.push_back(i);
     }
     AsmOut o;
-    st = run_assembly(e, p, r, glob, T, knots, 0, nullptr, nullptr, o);
+    st = run_assembly(e, p, r, glob, {&T}, knots, 0, nullptr, nullptr, o);
     if (st) return st;
     for (size_t i = 0; i < knots.size(); ++i) {
         u[kmap[i]] = o.kstats[i].B;
@@ -981,6 +884,12 @@ extern "C" pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_
                                   int32_t *iters, double *relres) {
     if (!e) return PDDS_ERR_INVALID;
     std::lock_guard<std::mutex> lk(e->mu);
+    return solve_system(e, n, row_ptr, cols, G, B, u, tol, maxit, iters, relres);
+}
+
+// (I + G) u = B on the device; the caller holds the engine lock
+pdds_status solve_system(pdds_engine *e, int32_t n, const int64_t *row_ptr, const int32_t *cols, const double *G,
+                         const double *B, double *u, double tol, int32_t maxit, int32_t *iters, double *relres) {
     if (n <= 0 || !row_ptr || !B || !u || (row_ptr[n] > 0 && (!cols || !G)) || !(tol >= 0.0))
         return fail(e, PDDS_ERR_INVALID, "bad system");
     const int64_t nnz = row_ptr[n];

@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     const int64_t item = a.item_base + (int64_t)blockIdx.x;
     const ItemDesc it = a.items[item];
     const KnotDesc kd = a.knots[it.knot];
-    const StencilDesc &sd = a.sten[kd.stencil];
+    const TableDesc tb = a.tables[kd.table];
+    const StencilDesc &sd = tb.sten[kd.stencil];
     const int64_t g0 = kd.path_off + it.p_begin;
     const double *rx = a.rec.x + g0, *ry = a.rec.y + g0, *rY = a.rec.Y + g0, *rs = a.rec.score + g0;
     const int8_t *rside = a.rec.side + g0;
@@ -147,8 +148,8 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
         // moments: unit u = (sub-batch b, entry e), entries fastest so that a wave reads one list
         for (int u = tid; u < nb * ne; u += WG) {
             const int b = u / ne, e = u - b * ne;
-            const int se = a.ent_side[sd.ent_off + e];
-            const double pxe = a.ent_x[sd.ent_off + e], pye = a.ent_y[sd.ent_off + e];
+            const int se = tb.ents[sd.ent_off + e];
+            const double pxe = tb.entx[sd.ent_off + e], pye = tb.enty[sd.ent_off + e];
             const double ic2 = sd.inv_c2[se];
             const uint16_t *L = lst + (b * 4 + se) * S;
             const int nl = s_lcnt[b * 4 + se];
@@ -165,14 +166,14 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
             double G[kNBG];
 #pragma unroll
             for (int b = 0; b < kNBG; ++b) G[b] = 0.0;
-            const int ec[2] = {a.csr_e1[sd.csr_off + c], a.csr_e2[sd.csr_off + c]};
+            const int ec[2] = {tb.e1[sd.csr_off + c], tb.e2[sd.csr_off + c]};
             for (int q = 0; q < 2; ++q) {
                 const int e = ec[q];
                 if (e < 0) continue;
-                const int se = a.ent_side[sd.ent_off + e];
+                const int se = tb.ents[sd.ent_off + e];
                 const int m = sd.msz[se];
                 const int el = e - eoff[se];
-                const double *A = a.ipsi + sd.ipsi_off[se];
+                const double *A = tb.ipsi + sd.ipsi_off[se];
                 double H[kNBG];
 #pragma unroll
                 for (int b = 0; b < kNBG; ++b) H[b] = 0.0;
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
     const int k = blockIdx.x;
     if (k >= f.n_knots) return;
     const KnotDesc kd = f.knots[k];
-    const StencilDesc &sd = f.sten[kd.stencil];
+    const StencilDesc &sd = f.tables[kd.table].sten[kd.stencil];
     const int nnz = sd.csr_nnz;
     const int64_t w0 = kd.item_begin, w1 = kd.item_begin + kd.item_count;
     const double N = (double)kd.n_paths;

@@ -173,21 +173,23 @@ struct Walker {
     uint32_t kid;
     int gmask;
     uint32_t g;            // batch path index (record slot; a batch holds fewer than 2^32 paths)
+    uint32_t kstream;      // Philox key word 1 of the path's knot
     R k2;                  // Helmholtz parameter of the path's knot
     bool active, fresh;
 
     __device__ __forceinline__ void init() {
-        pn1 = 0; pn2 = 0; kid = 0; gmask = 0; g = 0; k2 = 0;
+        pn1 = 0; pn2 = 0; kid = 0; kstream = 0; gmask = 0; g = 0; k2 = 0;
         active = false; fresh = false;
     }
 
     // take path gg of the batch; k is its knot
     __device__ __forceinline__ void start(const PathKernelArgs &a, int64_t gg, int k) {
         const KnotDesc &kd = a.knots[k];
-        const StencilDesc &sd = a.sten[kd.stencil];
+        const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
         for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
         gmask = sd.gmask;
         kid = kd.knot_id;
+        kstream = kd.stream;
         k2 = (R)kd.k2;
         g = (uint32_t)gg;
         s.path = (uint32_t)(gg - kd.path_off);
@@ -210,7 +212,7 @@ struct Walker {
     __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T, int it) {
         const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
         const bool stepping = active && !fresh;
-        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2, (it & 1) == 0);
+        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, kstream, kid, pn1, pn2, (it & 1) == 0);
         {
             PathState<R> t = s;
             em_step<R>(t, co, h, sqrth, pn1, pn2);
@@ -229,7 +231,7 @@ struct Walker {
         int kind;
         const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
         pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, k2, co);
-        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, a.stream, kid, pn1, pn2, (it & 1) != 0);
+        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, kstream, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);

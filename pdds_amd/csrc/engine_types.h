@@ -23,8 +23,10 @@ constexpr int kMaxQueues = 8;              // path queues: one per XCD of an MI3
 // One Monte Carlo knot of an assembly batch.
 struct KnotDesc {
     double x0, y0;          // start point
-    uint32_t knot_id;       // Philox counter word 2 (global knot index / request point id)
-    int32_t stencil;        // index into the stencil table
+    uint32_t knot_id;       // Philox counter word 2 (knot index within its request / mesh)
+    uint32_t stream;        // Philox key word 1 (the run's stream, or the serving request's)
+    int32_t table;          // stencil table of the knot (TableDesc index of the launch)
+    int32_t stencil;        // index into that table's stencils
     int64_t n_paths;
     int64_t path_off;       // global index of the knot's first path in the batch (ascending)
     int64_t out_off;        // first value of the knot's G row in the CSR value array
@@ -44,6 +46,16 @@ struct StencilDesc {
     int32_t csr_nnz;        // columns in the CSR row
     int32_t csr_off;        // first column map entry in csr_e1 / csr_e2
     int32_t gmask;          // bit s set: side s lies on the global boundary
+};
+
+// The device stencil tables of one mesh: a launch may carry knots of several meshes (serving),
+// each knot naming its table.
+struct TableDesc {
+    const StencilDesc *sten;
+    const double *entx, *enty;     // side-entry positions
+    const int8_t *ents;            // side of each entry
+    const int32_t *e1, *e2;        // contributing entries of each CSR column
+    const double *ipsi;            // Psi^-1 blocks
 };
 
 struct ItemDesc {
@@ -87,7 +99,7 @@ struct PatchDesc {
 struct PathKernelArgs {
     const KnotDesc *knots;
     int32_t n_knots;
-    const StencilDesc *sten;
+    const TableDesc *tables;
     int64_t total;          // paths in the batch
     int64_t base;           // plain launches: first path of this launch
     // persistent launches: queue q serves paths [q*total/n_queues, (q+1)*total/n_queues)
@@ -108,11 +120,7 @@ struct PathKernelArgs {
 
 struct ReduceArgs {
     const KnotDesc *knots;
-    const StencilDesc *sten;
-    const double *ent_x, *ent_y;
-    const int8_t *ent_side;
-    const int32_t *csr_e1, *csr_e2;
-    const double *ipsi;
+    const TableDesc *tables;
     const ItemDesc *items;
     int64_t item_base;
     Records rec;
@@ -124,7 +132,7 @@ struct ReduceArgs {
 
 struct FinalizeArgs {
     const KnotDesc *knots;
-    const StencilDesc *sten;
+    const TableDesc *tables;
     const ItemDesc *items;
     const ItemStat *stats;
     const double *partials;

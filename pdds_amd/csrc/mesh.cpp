@@ -2,6 +2,9 @@
 #include "mesh.h"
 
 #include <algorithm>
+#include <atomic>
+#include <map>
+#include <thread>
 #include <cmath>
 
 #include "pdds_math.h"
@@ -129,27 +132,67 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
         int kd;
         on_boundary[q] = std::fabs(rect_dist<double>(glob, x0[q], y0[q], ex, ey, nx, ny, kd)) <= 1e-8;
     }
-    // stencils, deduplicated
-    stencil_of.assign(nknots_, -1);
-    for (int q = 0; q < nknots_; ++q) {
-        Stencil s = compute_stencil(q);
-        int found = -1;
-        for (int t = (int)stencils.size() - 1; t >= 0 && t >= (int)stencils.size() - 8; --t)
-            if (stencils[t].same_as(s)) { found = t; break; }
-        if (found < 0) {
-            for (int t = 0; t < (int)stencils.size(); ++t)
-                if (stencils[t].same_as(s)) { found = t; break; }
-        }
-        if (found < 0) { stencils.push_back(std::move(s)); found = (int)stencils.size() - 1; }
-        stencil_of[q] = found;
+    // stencils of every knot (independent, spread over host threads), then deduplicated in knot
+    // order: a knot takes the first earlier stencil equal to its own
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth_all = std::min<size_t>((size_t)(hw ? hw : 1), 16);
+    std::vector<Stencil> per_knot(nknots_);
+    {
+        std::atomic<int> nextq{0};
+        auto work = [&]() {
+            for (int q; (q = nextq.fetch_add(1)) < nknots_;) per_knot[q] = compute_stencil(q);
+        };
+        std::vector<std::thread> pool;
+        for (size_t i = 1; i < std::min<size_t>(nth_all, (size_t)nknots_ / 64 + 1); ++i) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
     }
-    // per stencil: Psi^-1, CSR row, column -> side entry maps, global-side mask
-    for (Stencil &s : stencils) {
-        for (int d = 0; d < 4; ++d) {
-            if (s.idx[d].empty()) continue;
-            s.cshape[d] = fac[s.kind[d] < 0 ? 0 : (s.kind[d] > 3 ? 3 : s.kind[d])];
-            if (!psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d])) { s.singular = true; any_singular = true; }
+    stencil_of.assign(nknots_, -1);
+    std::map<std::vector<int64_t>, int> seen;
+    for (int q = 0; q < nknots_; ++q) {
+        const Stencil &s = per_knot[q];
+        std::vector<int64_t> key;
+        for (int i = 0; i < 4; ++i) {
+            key.push_back((int64_t)f64_bits(s.rect[i]));
+            key.push_back(s.kind[i]);
+            key.push_back((int64_t)s.idx[i].size());
+            key.insert(key.end(), s.idx[i].begin(), s.idx[i].end());
         }
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            it = seen.emplace(std::move(key), (int)stencils.size()).first;
+            stencils.push_back(std::move(per_knot[q]));
+        }
+        stencil_of[q] = it->second;
+    }
+    per_knot.clear();
+    // per stencil side: Psi^-1 (independent dense inversions, spread over host threads)
+    std::vector<std::pair<int, int>> jobs;
+    for (int t = 0; t < (int)stencils.size(); ++t)
+        for (int d = 0; d < 4; ++d)
+            if (!stencils[t].idx[d].empty()) {
+                Stencil &s = stencils[t];
+                s.cshape[d] = fac[s.kind[d] < 0 ? 0 : (s.kind[d] > 3 ? 3 : s.kind[d])];
+                jobs.emplace_back(t, d);
+            }
+    std::vector<char> ok(jobs.size(), 1);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+            Stencil &s = stencils[jobs[j].first];
+            const int d = jobs[j].second;
+            ok[j] = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]) ? 1 : 0;
+        }
+    };
+    const size_t nth = std::min<size_t>(nth_all, jobs.size() / 4 + 1);
+    std::vector<std::thread> pool;
+    for (size_t i = 1; i < nth; ++i) pool.emplace_back(work);
+    work();
+    for (std::thread &th : pool) th.join();
+    for (size_t j = 0; j < jobs.size(); ++j)
+        if (!ok[j]) { stencils[jobs[j].first].singular = true; any_singular = true; }
+    // per stencil: CSR row, column -> side entry maps, global-side mask
+    for (Stencil &s : stencils) {
         std::vector<int> ent;
         for (int d = 0; d < 4; ++d) ent.insert(ent.end(), s.idx[d].begin(), s.idx[d].end());
         s.cols = ent;

@@ -256,3 +256,75 @@ class Mesh:
             ipsi.append(a[:m * m].reshape(m, m))
         return {"rect": rect, "msz": msz, "kind": kind, "idx": idx[:t], "px": px[:t], "py": py[:t], "cond": cond,
                 "ipsi": ipsi}
+
+
+class Server:
+    """Continuous-batching request loop over an engine (pdds_server_*): submit point evaluations
+    and interface solves as they arrive, collect each result by ticket."""
+
+    def __init__(self, engine, problem, run, domain=(-1.0, -1.0, 1.0, 1.0), hbm_budget=0, max_inflight=0,
+                 prep_threads=0, round_paths=0):
+        cfg = A.ServerConfig()
+        cfg.problem = problem._c()
+        cfg.run = run
+        for i in range(4):
+            cfg.domain[i] = domain[i]
+        cfg.hbm_budget, cfg.max_inflight, cfg.prep_threads, cfg.round_paths = hbm_budget, max_inflight, prep_threads, round_paths
+        self._problem = problem          # keeps the LUT (if any) alive
+        self.engine = engine
+        h = C.c_void_p()
+        engine._chk(A.lib().pdds_server_create(engine.h, C.byref(cfg), C.byref(h)), "server create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            A.lib().pdds_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit_point(self, x, y, n_paths, k2=0.0, stream=1):
+        r = A.Request()
+        r.kind, r.stream, r.k2, r.n_paths, r.x, r.y = A.REQ_POINT, stream, k2, n_paths, x, y
+        return self._submit(r)
+
+    def submit_solve(self, n_sub, n_node, fac, n_paths, k2=0.0, stream=1, tol=1e-10):
+        r = A.Request()
+        r.kind, r.stream, r.k2, r.n_paths, r.n_sub, r.n_node, r.tol = A.REQ_SOLVE, stream, k2, n_paths, n_sub, n_node, tol
+        for i in range(4):
+            r.fac[i] = fac[i]
+        return self._submit(r)
+
+    def _submit(self, r):
+        t = C.c_uint64()
+        st = A.lib().pdds_server_submit(self.h, C.byref(r), C.byref(t))
+        if st != A.OK:
+            raise PddsError(st, "server submit")
+        return t.value
+
+    def _take(self, fn, ticket, cap):
+        res = A.Result()
+        u = np.zeros(max(cap, 1))
+        st = fn(self.h, ticket, C.byref(res), ptr(u), cap)
+        if st == A.ERR_NOT_READY:
+            return None
+        if st != A.OK:
+            raise PddsError(st, "server result")
+        out = res.as_dict()
+        if res.kind == A.REQ_SOLVE:
+            out["u_knots"] = u[:min(cap, res.n_knots)].copy()
+        return out
+
+    def poll(self, ticket, cap=1 << 16):
+        return self._take(A.lib().pdds_server_poll, ticket, cap)
+
+    def wait(self, ticket, cap=1 << 16):
+        return self._take(A.lib().pdds_server_wait, ticket, cap)
+
+    def stats(self):
+        v = [C.c_int64() for _ in range(4)]
+        b = C.c_double()
+        A.lib().pdds_server_stats(self.h, *[C.byref(x) for x in v], C.byref(b))
+        return {"submitted": v[0].value, "done": v[1].value, "rounds": v[2].value, "path_steps": v[3].value,
+                "busy_ms": b.value}

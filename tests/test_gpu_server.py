@@ -1,0 +1,124 @@
+"""Serving: the continuous-batching request loop (pdds_server_*).
+
+Each request's paths are keyed by its own Philox stream and every sum has a fixed order, so a
+request's result is the same bit for bit alone, in a mix of other requests, and as the direct
+pdds_point_eval / pdds_assemble + pdds_solve calls it stands for.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DOMAIN = (-1.0, -1.0, 1.0, 1.0)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from pdds_amd.engine import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _server(eng, precision="fp32", **kw):
+    from pdds_amd.engine import Server, bvp, default_run
+    return Server(eng, bvp("helmholtz"), default_run(h=2e-3, seed=23, precision=precision), DOMAIN, **kw)
+
+
+def _fac(n_sub, n_node, shape=1.0):
+    from pdds_amd.workloads import relative_fac
+    return relative_fac(DOMAIN[:2], DOMAIN[2:], n_sub, n_node, shape)
+
+
+def _mix(srv):
+    """Background traffic: points and solves of other streams and k^2."""
+    rng = np.random.default_rng(5)
+    tickets = []
+    for i in range(12):
+        x, y = rng.uniform(-0.9, 0.9, 2)
+        tickets.append(srv.submit_point(x, y, int(rng.integers(2000, 20000)), k2=float(rng.uniform(0, 16)),
+                                        stream=100 + i))
+    tickets.append(srv.submit_solve(3, 6, _fac(3, 6), 800, k2=4.0, stream=200))
+    tickets.append(srv.submit_solve(2, 9, _fac(2, 9), 600, k2=1.0, stream=201))
+    return tickets
+
+
+def test_point_same_alone_and_in_mix(eng):
+    srv = _server(eng)
+    alone = srv.wait(srv.submit_point(0.3, -0.2, 20000, k2=2.5, stream=7))
+    others = _mix(srv)
+    mixed_t = srv.submit_point(0.3, -0.2, 20000, k2=2.5, stream=7)
+    more = _mix(srv)
+    mixed = srv.wait(mixed_t)
+    for t in others + more:
+        r = srv.wait(t)
+        assert r["status"] == 0
+    srv.close()
+    assert alone["status"] == 0 and mixed["status"] == 0
+    assert alone["u"] == mixed["u"] and alone["u_var"] == mixed["u_var"]
+    assert alone["path_steps"] == mixed["path_steps"] > 0
+
+
+@pytest.mark.parametrize("round_paths", [0, 3000], ids=["one-round", "many-rounds"])
+def test_solve_same_alone_and_in_mix(eng, round_paths):
+    """Also with rounds much smaller than the solve, which is then spread over many rounds
+    shared with the other requests."""
+    srv = _server(eng, round_paths=round_paths)
+    fac = _fac(3, 8)
+    alone = srv.wait(srv.submit_solve(3, 8, fac, 1000, k2=3.0, stream=9))
+    others = _mix(srv)
+    t = srv.submit_solve(3, 8, fac, 1000, k2=3.0, stream=9)
+    more = _mix(srv)
+    mixed = srv.wait(t)
+    for o in others + more:
+        assert srv.wait(o)["status"] == 0
+    st = srv.stats()
+    srv.close()
+    assert alone["status"] == 0 and mixed["status"] == 0
+    assert alone["n_knots"] == mixed["n_knots"] > 0
+    assert np.array_equal(alone["u_knots"], mixed["u_knots"])
+    assert alone["iters"] == mixed["iters"]
+    assert st["done"] == st["submitted"]
+    if round_paths:
+        assert st["rounds"] > 10
+
+
+def test_point_matches_direct_point_eval(eng):
+    from pdds_amd.engine import bvp, default_run
+    srv = _server(eng, precision="fp64")
+    got = srv.wait(srv.submit_point(-0.4, 0.55, 8000, k2=6.0, stream=33))
+    srv.close()
+    run = default_run(h=2e-3, seed=23, precision="fp64", stream=33)
+    ref = eng.point_eval(bvp("helmholtz", k2=6.0), run, DOMAIN, [-0.4], [0.55], 8000)
+    assert got["u"] == ref["u"][0] and got["u_var"] == ref["u_var"][0]
+
+
+def test_solve_matches_direct_assembly_and_solve(eng):
+    from pdds_amd.engine import Mesh, bvp, default_run
+    n_sub, n_node, n = 3, 7, 2000
+    fac = _fac(n_sub, n_node)
+    srv = _server(eng, precision="fp64")
+    got = srv.wait(srv.submit_solve(n_sub, n_node, fac, n, k2=2.0, stream=44, tol=1e-12))
+    srv.close()
+    m = Mesh(DOMAIN[:2], DOMAIN[2:], n_sub, n_node, fac)
+    run = default_run(h=2e-3, seed=23, precision="fp64", stream=44)
+    out = eng.assemble(m, bvp("helmholtz", k2=2.0), run, n)
+    u, it, rr = eng.solve(m.n_knots, m.row_ptr, m.cols, out["G"], out["B"], tol=1e-12)
+    assert got["n_knots"] == m.n_knots
+    assert np.array_equal(got["u_knots"], u) and got["iters"] == it
+    # and the Feynman-Kac estimate of the manufactured solution is the right one
+    ex = bvp("helmholtz", k2=2.0).u(m.px, m.py)
+    assert np.sqrt(np.mean((u - ex) ** 2)) < 0.08
+
+
+def test_budget_and_timing_fields(eng):
+    from pdds_amd._capi import ERR_FULL, PddsError
+    srv = _server(eng, hbm_budget=1 << 20)
+    with pytest.raises(PddsError) as ei:
+        srv.submit_point(0.1, 0.1, 10 ** 6)          # 37 MB of exit records > 1 MiB
+    assert ei.value.status == ERR_FULL
+    r = srv.wait(srv.submit_point(0.1, 0.1, 5000, k2=1.0, stream=3))
+    b = srv.wait(srv.submit_point(1.0, 0.25, 5000))  # on the boundary: g, no Monte Carlo
+    srv.close()
+    assert r["status"] == 0 and 0.0 <= r["queue_ms"] <= r["latency_ms"] and r["path_steps"] > 0
+    assert b["path_steps"] == 0 and b["u_var"] == 0.0

@@ -89,6 +89,103 @@ def copy_calibration(n=1 << 30, reps=3):
     print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n, "bytes_written_per_copy": n, "copies": reps}}))
 
 
+def request_stream(duration_s, rate=200.0, seed=23, solve_paths=1000):
+    """The serving workload: Poisson arrivals (mean `rate` per second); 90 % point evaluations at a
+    uniform interior point with log-uniform 1e4..1e6 paths, 10 % interface solves of 4..16
+    subdomains per side and 16..64 knots per interface with `solve_paths` paths per knot; every
+    request with c = -k^2, k uniform in [0, 4]."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            return out
+        k2 = float(rng.uniform(0.0, 4.0)) ** 2
+        if rng.random() < 0.9:
+            x, y = rng.uniform(-0.999, 0.999, 2)
+            out.append({"kind": "point", "t": t, "x": float(x), "y": float(y), "n": int(10 ** rng.uniform(4, 6)),
+                        "k2": k2})
+        else:
+            out.append({"kind": "solve", "t": t, "n_sub": int(rng.integers(4, 17)), "n_node": int(rng.integers(16, 65)),
+                        "n": solve_paths, "k2": k2})
+
+
+def _pct(v, q):
+    return float(np.percentile(v, q)) if len(v) else None
+
+
+def serving_bench(eng, duration_s=1.5, h=1e-3, seed=23, plain=True):
+    """Serve the request stream with the continuous-batching server (FP32 paths, FP64 sums and
+    solves) and, for comparison, one request at a time with the plain FP64 GPU kernel.  Latency =
+    arrival to completion, including the host mesh build of a solve."""
+    from pdds_amd.engine import Mesh, Server, bvp, default_run
+    from pdds_amd.workloads import relative_fac
+    dom = (-1.0, -1.0, 1.0, 1.0)
+    reqs = request_stream(duration_s, seed=seed)
+    fac = {}
+    for r in reqs:
+        if r["kind"] == "solve":
+            fac[(r["n_sub"], r["n_node"])] = relative_fac(dom[:2], dom[2:], r["n_sub"], r["n_node"], 0.8)
+    srv = Server(eng, bvp("helmholtz"), default_run(h=h, seed=seed, precision="fp32"), dom, prep_threads=8)
+    t0 = time.perf_counter()
+    tickets = []
+    for i, r in enumerate(reqs):
+        d = t0 + r["t"] - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        if r["kind"] == "point":
+            tickets.append(srv.submit_point(r["x"], r["y"], r["n"], k2=r["k2"], stream=i + 1))
+        else:
+            tickets.append(srv.submit_solve(r["n_sub"], r["n_node"], fac[(r["n_sub"], r["n_node"])], r["n"],
+                                            k2=r["k2"], stream=i + 1))
+    res = [srv.wait(t, cap=1 << 20) for t in tickets]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = {"point": [x["latency_ms"] for x, r in zip(res, reqs) if r["kind"] == "point"],
+           "solve": [x["latency_ms"] for x, r in zip(res, reqs) if r["kind"] == "solve"]}
+    u_exact = bvp("helmholtz").u
+    perr = [abs(x["u"] - u_exact(r["x"], r["y"])) for x, r in zip(res, reqs) if r["kind"] == "point"]
+    out = {"workload": "Poisson arrivals 200/s for %.1f s (seed %d): %d point evaluations (1e4..1e6 paths), %d "
+                       "interface solves (4..16 subdomains, 16..64 knots/interface, 1000 paths/knot), c = -k^2 with "
+                       "k ~ U[0,4], h=%g, FP32 paths, FP64 sums and solves" % (duration_s, seed, len(lat["point"]),
+                                                                                len(lat["solve"]), h),
+           "requests": len(reqs), "failed": int(sum(x["status"] != 0 for x in res)),
+           "latency_ms": {"p50": _pct(lat["point"] + lat["solve"], 50), "p99": _pct(lat["point"] + lat["solve"], 99),
+                          "point_p50": _pct(lat["point"], 50), "point_p99": _pct(lat["point"], 99),
+                          "solve_p50": _pct(lat["solve"], 50), "solve_p99": _pct(lat["solve"], 99)},
+           "path_steps_per_s": st["path_steps"] / wall, "path_steps": st["path_steps"], "wall_s": wall,
+           "rounds": st["rounds"], "device_busy_s": st["busy_ms"] / 1e3,
+           "point_abs_error_rms": float(np.sqrt(np.mean(np.square(perr)))) if perr else None}
+    if plain:
+        # the same stream, one request at a time in arrival order, plain FP64 launches
+        lat_p = {"point": [], "solve": []}
+        steps = 0
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            d = t0 + r["t"] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            run = default_run(h=h, seed=seed, precision="fp64", plain=1, stream=i + 1)
+            prob = bvp("helmholtz", k2=r["k2"])
+            if r["kind"] == "point":
+                o = eng.point_eval(prob, run, dom, [r["x"]], [r["y"]], r["n"])
+            else:
+                m = Mesh(dom[:2], dom[2:], r["n_sub"], r["n_node"], fac[(r["n_sub"], r["n_node"])])
+                o = eng.assemble(m, prob, run, r["n"])
+                eng.solve(m.n_knots, m.row_ptr, m.cols, o["G"], o["B"], tol=1e-10)
+            steps += o["counters"]["path_steps"]
+            lat_p[r["kind"]].append(1e3 * (time.perf_counter() - (t0 + r["t"])))
+        wall_p = time.perf_counter() - t0
+        out["plain_gpu"] = {"latency_ms": {"p50": _pct(lat_p["point"] + lat_p["solve"], 50),
+                                           "p99": _pct(lat_p["point"] + lat_p["solve"], 99),
+                                           "point_p50": _pct(lat_p["point"], 50), "point_p99": _pct(lat_p["point"], 99),
+                                           "solve_p50": _pct(lat_p["solve"], 50), "solve_p99": _pct(lat_p["solve"], 99)},
+                            "path_steps_per_s": steps / wall_p, "wall_s": wall_p,
+                            "form": "one request at a time in arrival order, plain FP64 kernel (one path per lane)"}
+    return out
+
+
 def rms_error(eng, mesh, prob, out):
     u, it, rr = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, out["G"], out["B"], tol=1e-12)
     ex = prob.u(mesh.px, mesh.py)
@@ -104,6 +201,9 @@ def main():
     ap.add_argument("--paths", type=int, default=0, help="override paths per knot (0: the config's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-only", action="store_true", help="print only the serving scenario")
+    ap.add_argument("--serving-seconds", type=float, default=1.5)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_cfg2.json"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (the reference kernel for HBM counters)")
@@ -122,6 +222,10 @@ def main():
 
     eng = Engine(0)
     info = eng.info()
+    if args.serving_only:
+        print(json.dumps({"serving": serving_bench(eng, args.serving_seconds)}))
+        eng.close()
+        return
     mesh, prob, n_paths, h, seed = W.build(args.config, eng)
     if args.paths:
         n_paths = args.paths
@@ -200,6 +304,8 @@ def main():
         extras["engine_fp64"] = {"path_steps_per_s": value, "seconds": wall / args.steps, "rms_interface_error": rms}
         extras["engine_fp32_vs_plain_fp64"] = extras["engine_fp32"]["path_steps_per_s"] / extras["plain_fp64"]["path_steps_per_s"]
         result["forms"] = extras
+    if not args.no_serving and not args.no_extras:
+        result["serving"] = serving_bench(eng, args.serving_seconds)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(mesh, {3: "varcoef_lut", 0: "monegros", 1: "sinsin"}[prob.pde],
                                               n_paths, h, seed)

@@ -230,6 +230,9 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             off += k.n;
         }
         items_all += (int64_t)items.size();
+        // knots of other streams or k2 than the run's (a serving round): per-lane values
+        bool mixed = false;
+        for (int q = 0; q < nk; ++q) mixed = mixed || kd[q].stream != r->stream || kd[q].k2 != pp.k2;
         const size_t shmem = reduce_lds_bytes(ne_cap, nnz_cap, S);
         if (shmem > 160 * 1024 || S > 65535)
             return fail(e, PDDS_ERR_UNSUPPORTED, "stencil or sub-batch too large for the LDS reduction buffers");
@@ -308,11 +311,11 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             const int g = persistent ? grid : (int)std::min<int64_t>(grid, (tot - base + wg - 1) / wg);
             hipError_t lr;
             if (tiled)
-                lr = r->precision == PDDS_FP64 ? launch_path_tiled_f64(g, lds_bytes, e->stream, pa)
-                                               : launch_path_tiled_f32(g, lds_bytes, e->stream, pa);
+                lr = r->precision == PDDS_FP64 ? launch_path_tiled_f64(mixed, g, lds_bytes, e->stream, pa)
+                                               : launch_path_tiled_f32(mixed, g, lds_bytes, e->stream, pa);
             else
-                lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, g, wg, e->stream, pa)
-                                               : launch_path_f32(prob->pde, persistent, g, wg, e->stream, pa);
+                lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, mixed, g, wg, e->stream, pa)
+                                               : launch_path_f32(prob->pde, persistent, mixed, g, wg, e->stream, pa);
             if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("path launch: ") + hipGetErrorString(lr));
         }
         HIPCHK(hipEventRecord(e->ev[1], e->stream));

@@ -6,33 +6,41 @@
 namespace pdds {
 
 template <int PDE>
-static hipError_t path_f32(bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
-    if (persistent)
+static hipError_t path_f32(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
+    if (!persistent)
+        hipLaunchKernelGGL((plain_kernel<float, PDE>), dim3(grid), dim3(wg), 0, s, a);
+    else if (mixed)
         hipLaunchKernelGGL((path_kernel<float, PDE, true>), dim3(grid), dim3(wg), 0, s, a);
     else
         hipLaunchKernelGGL((path_kernel<float, PDE, false>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
+hipError_t launch_path_f32(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+                           const PathKernelArgs &a) {
     switch (pde) {
-        case PDE_MONEGROS: return path_f32<PDE_MONEGROS>(persistent, grid, wg, s, a);
-        case PDE_SINSIN: return path_f32<PDE_SINSIN>(persistent, grid, wg, s, a);
-        case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(persistent, grid, wg, s, a);
-        case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(persistent, grid, wg, s, a);
-        case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(persistent, grid, wg, s, a);
+        case PDE_MONEGROS: return path_f32<PDE_MONEGROS>(persistent, mixed, grid, wg, s, a);
+        case PDE_SINSIN: return path_f32<PDE_SINSIN>(persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(persistent, mixed, grid, wg, s, a);
+        case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(persistent, mixed, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+template <bool MIXED>
+static hipError_t path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float>,
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float, MIXED>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(path_kernel_tiled<float>, dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED>), dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_path_tiled_f32(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    return mixed ? path_tiled_f32<true>(grid, lds, s, a) : path_tiled_f32<false>(grid, lds, s, a);
 }
 
 template <int PDE>

@@ -161,10 +161,37 @@ __device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int
     return lo;
 }
 
+// Exit record of a stopped path at batch slot g: the exit projection, Y, the score
+// (Z + xi, plus Y g(X) for a global-boundary exit, Reduce_Analytic
+// Solvers/EMFKACSolver.hpp:559-566), the exit side and the step count (kCappedBit: stopped by
+// the step cap while still inside).
+template <class R, int PDE>
+__device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, const PathState<R> &s,
+                                           const R rect[4], int gmask, R ex, R ey, int kind, bool capped_inside) {
+    int side;
+    if constexpr (sizeof(R) == 8) side = classify_coord(rect, a.glob, (double)ex, (double)ey);
+    else side = classify_kind(kind, gmask);
+    double sc;
+    if (side == 4) {
+        R gv = pde_g<PDE, R>(ex, ey, a.pp);
+        sc = (double)(s.Z + s.Y * gv + s.xi);
+    } else {
+        sc = (double)(s.Z + s.xi);
+    }
+    a.rec.x[g] = (double)ex;
+    a.rec.y[g] = (double)ey;
+    a.rec.Y[g] = (double)s.Y;
+    a.rec.score[g] = sc;
+    a.rec.side[g] = (int8_t)side;
+    a.rec.steps[g] = (int32_t)s.st | (capped_inside ? kCappedBit : 0);
+}
+
 // ----------------------------------------------------------------------------
 // Per-lane walker: the state of the path a lane carries and one step of it
 // ----------------------------------------------------------------------------
-template <class R, int PDE>
+// MIXED: the knots of the launch differ in Philox stream or Helmholtz k2 (serving rounds), so
+// each lane carries its knot's; otherwise both are the launch's and stay in scalar registers.
+template <class R, int PDE, bool MIXED>
 struct Walker {
     PathState<R> s;
     Coef<R> co;            // coefficients at the current X
@@ -189,8 +216,10 @@ struct Walker {
         for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
         gmask = sd.gmask;
         kid = kd.knot_id;
-        kstream = kd.stream;
-        k2 = (R)kd.k2;
+        if constexpr (MIXED) {
+            kstream = kd.stream;
+            k2 = (R)kd.k2;
+        }
         g = (uint32_t)gg;
         s.path = (uint32_t)(gg - kd.path_off);
         s.x = (R)kd.x0; s.y = (R)kd.y0;
@@ -211,8 +240,10 @@ struct Walker {
     template <bool TILED, bool DRAW_FIRST>
     __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T, int it) {
         const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
+        const uint32_t strm = MIXED ? kstream : a.stream;
+        const R kk2 = MIXED ? k2 : (R)a.pp.k2;
         const bool stepping = active && !fresh;
-        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, kstream, kid, pn1, pn2, (it & 1) == 0);
+        if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) == 0);
         {
             PathState<R> t = s;
             em_step<R>(t, co, h, sqrth, pn1, pn2);
@@ -230,30 +261,15 @@ struct Walker {
         R ex, ey, nx, ny;
         int kind;
         const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, k2, co);
-        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, kstream, kid, pn1, pn2, (it & 1) != 0);
+        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, kk2, co);
+        if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
         const bool go = d < beta * nrm * sqrth;
         const bool capped = (int64_t)s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
-            int side;
-            if constexpr (sizeof(R) == 8) side = classify_coord(rect, a.glob, (double)ex, (double)ey);
-            else side = classify_kind(kind, gmask);
-            double sc;
-            if (side == 4) {
-                R gv = pde_g<PDE, R>(ex, ey, a.pp);
-                sc = (double)(s.Z + s.Y * gv + s.xi);
-            } else {
-                sc = (double)(s.Z + s.xi);
-            }
-            a.rec.x[g] = (double)ex;
-            a.rec.y[g] = (double)ey;
-            a.rec.Y[g] = (double)s.Y;
-            a.rec.score[g] = sc;
-            a.rec.side[g] = (int8_t)side;
-            a.rec.steps[g] = (int32_t)s.st | ((capped && go) ? kCappedBit : 0);
+            write_exit<R, PDE>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
             active = false;
         }
         fresh = false;
@@ -307,29 +323,24 @@ struct WavePool {
 };
 
 // ----------------------------------------------------------------------------
-// Path kernel: one path per lane, persistent (XCD queues) or plain (one launch lane per path)
+// Persistent path kernel: one path per lane, lanes refilled from XCD-local queues
 // ----------------------------------------------------------------------------
-template <class R, int PDE, bool PERSIST>
+template <class R, int PDE, bool MIXED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
     const int lane = __lane_id();
     const LutTile none{nullptr, 0, 0, 0, 0};
-    Walker<R, PDE> w;
+    Walker<R, PDE, MIXED> w;
     w.init();
     bool done = false;
-    if (!PERSIST) {
-        const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (gg < a.total) w.start(a, gg, find_knot(a.knots, a.n_knots, gg));
-        else done = true;
-    }
     // The wave starts on its XCD's queue, so the paths (and LUT texels) of one XCD's L2
     // stay in one contiguous range of knots; a drained queue sends the wave on to the next.
     WavePool pool;
     bool drained = false;
-    int q = PERSIST ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
+    int q = (int)(blockIdx.x % (unsigned)a.n_queues);
     int tried = 0;
     for (int it = 1;; ++it) {
         const bool can_start = may_start<R>(it);
-        if (PERSIST && can_start) {
+        if (can_start) {
             const bool need = !w.active && !done;
             const unsigned long long m = __ballot(need);
             if (m) {
@@ -342,11 +353,52 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
             }
         }
         if (!__any(w.active)) {
-            if (can_start || !PERSIST) break;
+            if (can_start) break;
             continue;   // idle until the next iteration on which paths may start
         }
         w.template step<false, false>(a, none, it);
     }
+}
+
+// ----------------------------------------------------------------------------
+// Plain path kernel, the baseline GPU form (the shape of the reference's CUDA SolveLoop without
+// its chunked relaunches, Solvers/SolveLoop.cuh:20-887): one work-item per path of the batch,
+// each stepping its own path to the exit; a wave lasts as long as its slowest path.
+// ----------------------------------------------------------------------------
+template <class R, int PDE>
+__global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
+    const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gg >= a.total) return;
+    const KnotDesc &kd = a.knots[find_knot(a.knots, a.n_knots, gg)];
+    const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
+    const LutTile none{nullptr, 0, 0, 0, 0};
+    const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta, k2 = (R)kd.k2;
+    R rect[4];
+    for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
+    PathState<R> s;
+    s.path = (uint32_t)(gg - kd.path_off);
+    s.x = (R)kd.x0; s.y = (R)kd.y0;
+    s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
+    s.st = 0;
+    Coef<R> co;
+    pde_coef_t<PDE, R, false>(s.x, s.y, a.pp, none, k2, co);
+    R ex, ey;
+    int kind;
+    bool go, capped;
+    do {
+        R n1, n2, nx, ny;
+        draw_normals<R>(s, a.seed, kd.stream, kd.knot_id, n1, n2, (s.st & 1) == 0);
+        em_step<R>(s, co, h, sqrth, n1, n2);
+        s.st += 1;
+        const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        pde_coef_t<PDE, R, false>(s.x, s.y, a.pp, none, k2, co);
+        const R ga = nx * co.s00 + ny * co.s10;
+        const R gb = nx * co.s01 + ny * co.s11;
+        const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
+        go = d < beta * nrm * sqrth;
+        capped = (int64_t)s.st >= a.max_steps;
+    } while (go && !capped);
+    write_exit<R, PDE>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind, capped && go);
 }
 
 // ----------------------------------------------------------------------------
@@ -368,13 +420,13 @@ __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) 
     return lo;
 }
 
-template <class R>
+template <class R, bool MIXED>
 __global__ void __launch_bounds__(tiled_wg<R>()) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE_VARCOEF_LUT>())))
 path_kernel_tiled(PathKernelArgs a) {
     extern __shared__ float4 tile[];
     __shared__ int s_patch;
     const int lane = __lane_id();
-    Walker<R, PDE_VARCOEF_LUT> w;
+    Walker<R, PDE_VARCOEF_LUT, MIXED> w;
     w.init();
     // first patch: the one holding this work-group's share of the batch's path range
     int p = find_patch(a.patches, a.n_patches, (int64_t)blockIdx.x * a.total / (int64_t)gridDim.x);

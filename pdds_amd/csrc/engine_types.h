@@ -158,11 +158,13 @@ struct PathsArgs {
 };
 
 // Launchers (defined in engine_f64.hip / engine_f32.hip)
-hipError_t launch_path_tiled_f64(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
-hipError_t launch_path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_tiled_f64(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_tiled_f32(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
 int path_tiled_wg(int precision);
-hipError_t launch_path_f64(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
-hipError_t launch_path_f32(int pde, bool persistent, int grid, int wg, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_f64(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+                           const PathKernelArgs &a);
+hipError_t launch_path_f32(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+                           const PathKernelArgs &a);
 hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a);
 hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);
 hipError_t launch_paths_f32(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);

@@ -327,7 +327,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         const int64_t nit = (int64_t)items.size();
         for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
             ra.item_base = base;
-            hipError_t lr = launch_reduce((int)std::min<int64_t>(nit - base, (int64_t)1 << 30), 256, shmem, e->stream, ra);
+            hipError_t lr = launch_reduce((int)std::min<int64_t>(nit - base, (int64_t)1 << 30),
+                                          kReduceWG, shmem, e->stream, ra);
             if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("reduce launch: ") + hipGetErrorString(lr));
         }
         FinalizeArgs f;

@@ -68,16 +68,20 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 // One work-group per work item: per-side RBF moments per sub-batch, G_beta = -Psi^-T Phi per
 // CSR column, sums in fixed order (sub-batch -> item), B sums per sub-batch.  A group of kNBG
 // sub-batches is staged in LDS with, per (sub-batch, side), the list of its paths that exit
-// through that side in path order; the moment of entry e in sub-batch b then loops over that
-// list only, adding in path order (the order of the per-exit loop it replaces).
-__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
-    extern __shared__ double smem[];
+// through that side in path order (built with wave ballots).  A thread then owns one stencil
+// entry e and, for each sub-batch b, adds Y phi_e(X) over the list of e's side in path order
+// (the order of the per-exit loop it replaces), two exits per iteration so that two exp chains
+// are in flight; Phi is kept [entry][sub-batch] so that the Psi^-T pass reads a row of kNBG
+// values with two 16-byte LDS reads per pair of sub-batches.
+__global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
+    extern __shared__ __align__(16) double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
     __shared__ int s_lcnt[kNBG * 4];
     __shared__ unsigned long long s_steps;
     __shared__ unsigned int s_cnt[8];
     const int tid = threadIdx.x;
     const int WG = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = WG >> 6;
     const int64_t item = a.item_base + (int64_t)blockIdx.x;
     const ItemDesc it = a.items[item];
     const KnotDesc kd = a.knots[it.knot];
@@ -89,13 +93,14 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
     const int32_t *rst = a.rec.steps + g0;
     const int S = a.S;
     const int GS = kNBG * S;
-    double *Phi = smem;
+    double *Phi = smem;                                     // [ne_cap][kNBG]
     double *accS = smem + kNBG * a.ne_cap;
     double *accQ = accS + a.nnz_cap;
     double *gx = accQ + a.nnz_cap;
     double *gy = gx + GS;
     double *gY = gy + GS;
-    uint16_t *lst = (uint16_t *)(gY + GS);
+    double *gs = gY + GS;
+    uint16_t *lst = (uint16_t *)(gs + GS);
     int8_t *gside = (int8_t *)(lst + 4 * GS);
     if (tid == 0) {
         s_steps = 0ULL;
@@ -132,42 +137,63 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
             gx[p] = rx[q0 + p];
             gy[p] = ry[q0 + p];
             gY[p] = rY[q0 + p];
+            gs[p] = rs[q0 + p];
             gside[p] = rside[q0 + p];
         }
         __syncthreads();
-        // per (sub-batch, side) lists in path order, and the B sums of each sub-batch
-        if (tid < nb * 4) {
-            const int b = tid >> 2, q = tid & 3;
+        // per (sub-batch, side) lists in path order: a wave per sub-batch, 64 paths per ballot
+        for (int b = wave; b < nb; b += nwaves) {
             const int p0 = b * S, p1 = (p0 + S) < gn ? (p0 + S) : gn;
-            int c = 0;
-            for (int p = p0; p < p1; ++p)
-                if (gside[p] == q) lst[(b * 4 + q) * S + c++] = (uint16_t)(p - p0);
-            s_lcnt[b * 4 + q] = c;
-        } else if (tid >= 64 && tid < 64 + nb) {
-            const int b = tid - 64;
-            const int p0 = q0 + b * S;
-            const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
-            double bs = 0.0, bb = 0.0;
-            for (int p = p0; p < p1; ++p) { const double v = rs[p]; bs = bs + v; bb = bb + v * v; }
-            s_bsb[b] = bs;
-            s_bbsb[b] = bb;
+            int cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0;
+            const unsigned long long below = (1ULL << lane) - 1ULL;
+            for (int c0 = p0; c0 < p1; c0 += 64) {
+                const int p = c0 + lane;
+                const int sdp = p < p1 ? (int)gside[p] : -1;
+                uint16_t *Lb = lst + (b * 4) * S;
+                const unsigned long long m0 = __ballot(sdp == 0), m1 = __ballot(sdp == 1);
+                const unsigned long long m2 = __ballot(sdp == 2), m3 = __ballot(sdp == 3);
+                if (sdp == 0) Lb[0 * S + cnt0 + __popcll(m0 & below)] = (uint16_t)(p - p0);
+                if (sdp == 1) Lb[1 * S + cnt1 + __popcll(m1 & below)] = (uint16_t)(p - p0);
+                if (sdp == 2) Lb[2 * S + cnt2 + __popcll(m2 & below)] = (uint16_t)(p - p0);
+                if (sdp == 3) Lb[3 * S + cnt3 + __popcll(m3 & below)] = (uint16_t)(p - p0);
+                cnt0 += __popcll(m0); cnt1 += __popcll(m1); cnt2 += __popcll(m2); cnt3 += __popcll(m3);
+            }
+            if (lane == 0) {
+                s_lcnt[b * 4 + 0] = cnt0; s_lcnt[b * 4 + 1] = cnt1;
+                s_lcnt[b * 4 + 2] = cnt2; s_lcnt[b * 4 + 3] = cnt3;
+            } else if (lane == 1) {
+                // B sums of the sub-batch in path order
+                double bs = 0.0, bb = 0.0;
+                for (int p = p0; p < p1; ++p) { const double v = gs[p]; bs = bs + v; bb = bb + v * v; }
+                s_bsb[b] = bs;
+                s_bbsb[b] = bb;
+            }
         }
         __syncthreads();
-        // moments: unit u = (sub-batch b, entry e), entries fastest so that a wave reads one list
-        for (int u = tid; u < nb * ne; u += WG) {
-            const int b = u / ne, e = u - b * ne;
+        // moments Phi[e][b] = sum over the exits of e's side in sub-batch b of Y phi_e(X)
+        for (int e = tid; e < ne; e += WG) {
             const int se = tb.ents[sd.ent_off + e];
             const double pxe = tb.entx[sd.ent_off + e], pye = tb.enty[sd.ent_off + e];
             const double ic2 = sd.inv_c2[se];
-            const uint16_t *L = lst + (b * 4 + se) * S;
-            const int nl = s_lcnt[b * 4 + se];
-            const int pb = b * S;
-            double acc = 0.0;
-            for (int l = 0; l < nl; ++l) {
-                const int p = pb + L[l];
-                acc = acc + gY[p] * rbf(pxe, pye, gx[p], gy[p], ic2);
+            for (int b = 0; b < nb; ++b) {
+                const uint16_t *L = lst + (b * 4 + se) * S;
+                const int nl = s_lcnt[b * 4 + se];
+                const int pb = b * S;
+                double acc = 0.0;
+                int l = 0;
+                for (; l + 1 < nl; l += 2) {
+                    const int pa = pb + L[l], pc = pb + L[l + 1];
+                    const double fa = gY[pa] * rbf(pxe, pye, gx[pa], gy[pa], ic2);
+                    const double fc = gY[pc] * rbf(pxe, pye, gx[pc], gy[pc], ic2);
+                    acc = acc + fa;
+                    acc = acc + fc;
+                }
+                if (l < nl) {
+                    const int pa = pb + L[l];
+                    acc = acc + gY[pa] * rbf(pxe, pye, gx[pa], gy[pa], ic2);
+                }
+                Phi[e * kNBG + b] = acc;
             }
-            Phi[b * a.ne_cap + e] = acc;
         }
         __syncthreads();
         for (int c = tid; c < nnz; c += WG) {
@@ -181,16 +207,35 @@ __global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a) {
                 const int se = tb.ents[sd.ent_off + e];
                 const int m = sd.msz[se];
                 const int el = e - eoff[se];
-                const double *A = tb.ipsi + sd.ipsi_off[se];
+                const double *A = tb.ipsi + sd.ipsi_off[se] + el;
+                const double2 *ph = (const double2 *)(Phi + eoff[se] * kNBG);
                 double H[kNBG];
 #pragma unroll
                 for (int b = 0; b < kNBG; ++b) H[b] = 0.0;
-                for (int j = 0; j < m; ++j) {
-                    const double aj = A[(int64_t)j * m + el];
-                    const double *ph = Phi + eoff[se] + j;
+                // Psi^-1 column el, eight rows per batch of loads so that their latency overlaps
+                int j = 0;
+                for (; j + 8 <= m; j += 8) {
+                    double av[8];
 #pragma unroll
-                    for (int b = 0; b < kNBG; ++b)
-                        if (b < nb) H[b] = H[b] + aj * ph[b * a.ne_cap];
+                    for (int u = 0; u < 8; ++u) av[u] = A[(int64_t)(j + u) * m];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+#pragma unroll
+                        for (int b2 = 0; b2 < kNBG / 2; ++b2) {
+                            const double2 v = ph[(j + u) * (kNBG / 2) + b2];
+                            H[2 * b2] = H[2 * b2] + av[u] * v.x;
+                            H[2 * b2 + 1] = H[2 * b2 + 1] + av[u] * v.y;
+                        }
+                    }
+                }
+                for (; j < m; ++j) {
+                    const double aj = A[(int64_t)j * m];
+#pragma unroll
+                    for (int b2 = 0; b2 < kNBG / 2; ++b2) {
+                        const double2 v = ph[j * (kNBG / 2) + b2];
+                        H[2 * b2] = H[2 * b2] + aj * v.x;
+                        H[2 * b2 + 1] = H[2 * b2 + 1] + aj * v.y;
+                    }
                 }
 #pragma unroll
                 for (int b = 0; b < kNBG; ++b) G[b] = G[b] + (-H[b]);

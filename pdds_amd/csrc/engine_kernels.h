@@ -480,7 +480,7 @@ path_kernel_tiled(PathKernelArgs a) {
 // ----------------------------------------------------------------------------
 // Reduction kernel: one work-group per work item
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) reduce_kernel(ReduceArgs a);
+__global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a);
 
 // One block per knot: sum the knot's items in item order.
 __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);

@@ -6,15 +6,16 @@
 namespace pdds {
 
 constexpr int kNBG = 8;                    // sub-batches reduced per pass over Psi^-1
+constexpr int kReduceWG = 256;            // reduction work-group
 
-// LDS of the reduction kernel: Phi [kNBG][ne_cap], accS/accQ [nnz_cap] (FP64), the group's
-// records x, y, Y [kNBG*S] (FP64), their sides [kNBG*S] (int8) and per-(sub-batch, side)
-// path lists [kNBG][4][S] (uint16)
+// LDS of the reduction kernel: Phi [ne_cap][kNBG], accS/accQ [nnz_cap] (FP64), the group's
+// records x, y, Y and score [kNBG*S] (FP64), their sides [kNBG*S] (int8) and per-(sub-batch,
+// side) path lists [kNBG][4][S] (uint16)
 inline size_t reduce_lds_bytes(int ne_cap, int nnz_cap, int S) {
     const size_t g = (size_t)kNBG * S;
-    size_t b = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap + 3 * g) * sizeof(double);
+    size_t b = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap + 4 * g) * sizeof(double);
     b += g * 4 * sizeof(uint16_t) + g;
-    return (b + 7) & ~(size_t)7;
+    return (b + 15) & ~(size_t)15;
 }
 constexpr int32_t kCappedBit = 1 << 30;    // record flag: path stopped by the step cap
 constexpr int64_t kPoolChunk = 256;        // paths a wave takes from a path queue at once

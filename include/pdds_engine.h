@@ -86,6 +86,8 @@ typedef struct {
     int32_t lut_lds;        /* 1: stage LUT tiles in LDS when they fit (default 1) */
     int32_t plain;          /* 1: plain launch -- one path per lane, one work-group per item of wg_size
                                paths, no lane recycling, no persistent queue (the baseline GPU form) */
+    int32_t paths_per_lane; /* persistent launches: a wave takes 64 x this many paths from its queue
+                               at once (0: 4); results do not depend on it */
 } pdds_run;
 void pdds_run_defaults(pdds_run *r);
 
@@ -154,6 +156,68 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
 pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
                           const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var,
                           double *B, double *BB, pdds_counters *ctr);
+
+/* ---- assembly plus each knot's equation statistics ----
+   As pdds_assemble.  With u_ref (n_knots interface values, e.g. a previous solve) it also returns,
+   per Monte Carlo knot k, the mean (eq_mean, may be NULL) and per-path variance (eq_var) of the
+   knot's equation estimator B_k - sum_j G_kj u_ref_j, by batch means over the sub-batches;
+   boundary knots get g and 0.  sqrt(eq_var / N_k) is the standard error of knot k's equation.
+   Replaces the variance inputs of the commented Compute_h_N
+   (PDDAlgorithms/PDDSparseGM.cpp:4338-4395). */
+pdds_status pdds_assemble_eq(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                             const int64_t *paths_per_knot, int64_t n_paths, const double *u_ref, double *G,
+                             double *G_var, double *B, double *BB, double *eq_mean, double *eq_var,
+                             pdds_counters *ctr);
+
+/* ---- variance-targeted path counts (batching form; the reference's policy is commented out,
+   PDDAlgorithms/PDDSparseGM.cpp:4358-4365) ----
+   A pilot assembly and solve, then rounds that raise each knot's path count to
+   growth * eq_var_k / se_target^2 (within [n_min, n_max]) until every knot's equation standard
+   error is at most se_target or its count is n_max.  Every round is one assembly of all knots with
+   their uneven counts in one persistent launch, followed by the device solve.  Outputs: u (n_knots),
+   the final counts (0 for boundary knots) and, if se is not NULL, each knot's standard error. */
+typedef struct {
+    double se_target;       /* target standard error of every knot equation (default 1e-3) */
+    int64_t n_min, n_max;   /* per-knot path-count bounds (defaults 1e3, 1e6) */
+    int64_t n_pilot;        /* pilot paths per knot (0: n_min) */
+    int32_t max_rounds;     /* refinement rounds after the pilot (default 3) */
+    double tol;             /* BiCGSTAB relative tolerance (default 1e-10) */
+    double growth;          /* safety factor on the variance-targeted count (default 1.1) */
+} pdds_adaptive;
+typedef struct {
+    int32_t rounds;         /* refinement rounds run */
+    int32_t iters;          /* BiCGSTAB iterations of the final solve */
+    double relres;          /* and its relative residual */
+    int64_t paths;          /* paths of the final assembly */
+    int64_t path_steps;     /* path-steps of every assembly, pilots included */
+    double max_se;          /* largest knot standard error of the final assembly */
+    int32_t knots_capped;   /* knots held at n_max above the target */
+    double kernel_ms;       /* device time of the path and reduction kernels, all assemblies */
+    double total_ms;        /* host wall time */
+} pdds_adaptive_report;
+void pdds_adaptive_defaults(pdds_adaptive *c);
+pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                                const pdds_adaptive *cfg, double *u, int64_t *paths_per_knot, double *se,
+                                pdds_adaptive_report *rep);
+
+/* ---- launch autotuner (no counterpart in the reference) ----
+   Times trial assemblies of a sample of the mesh's knots (every k-th Monte Carlo knot, at most
+   max_knots of them, sample_paths paths each) over the launch shapes that do not change results
+   -- LDS tiling on or off, work-group size, work-groups per CU, paths per lane -- and writes into
+   *best the base run with the fastest shape (path-steps per second of the whole assembly). */
+typedef struct {
+    int64_t sample_paths;   /* paths per sampled knot (0: 4096) */
+    int32_t max_knots;      /* sampled knots (0: 256) */
+    int32_t reps;           /* timed repetitions per candidate, after one warm-up (0: 2) */
+} pdds_tune_config;
+typedef struct {
+    int32_t candidates;     /* shapes timed */
+    double best_rate;       /* path-steps/s of the chosen shape */
+    double base_rate;       /* path-steps/s of the base run's own shape */
+    double total_ms;        /* host wall time of the search */
+} pdds_tune_report;
+pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *base,
+                          const pdds_tune_config *cfg, pdds_run *best, pdds_tune_report *rep);
 
 /* ---- Feynman-Kac estimate of u at n points of the global rectangle `glob` ---- */
 pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],

@@ -424,12 +424,49 @@ int orc_reduce_faithful(const orc_problem *pb, int64_t npaths, const double *xt,
     return nnz;
 }
 
+/* Sum over a row's columns of v[k] = G_b,k * w[k] in the engine's reduction order: column k to
+   thread k mod 256 of a 256-thread work-group (each thread adding its columns in increasing k),
+   a xor butterfly (offsets 32, 16, ..., 1) inside each 64-lane wave, then the four waves in
+   order.  The order is the engine's own design, restated so that its sums compare bit for bit. */
+static double orc_wg_sum(const double *G, const double *w, int nnz) {
+    double part[256];
+    for (int t = 0; t < 256; ++t) {
+        part[t] = 0.0;
+        for (int k = t; k < nnz; k += 256) part[t] = part[t] + G[k] * w[k];
+    }
+    double T = 0.0;
+    for (int wv = 0; wv < 4; ++wv) {
+        double v[64], nv[64];
+        for (int i = 0; i < 64; ++i) v[i] = part[wv * 64 + i];
+        for (int off = 32; off > 0; off >>= 1) {
+            for (int i = 0; i < 64; ++i) nv[i] = v[i] + v[i ^ off];
+            memcpy(v, nv, sizeof(v));
+        }
+        T = T + v[0];
+    }
+    return T;
+}
+
 int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
                        const double *yt, const double *Y, const double *Z, const double *xi,
                        const double rect[4], const double glob[4], const int msz[4],
                        const int *idx, const double *px, const double *py, const double cshape[4],
                        const double *ipsi, int quirk,
                        int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6]) {
+    return orc_reduce_batched_eq(pb, npaths, S, P, xt, yt, Y, Z, xi, rect, glob, msz, idx, px, py, cshape, ipsi,
+                                 quirk, NULL, cols, G, Gvar, B, BB, cnt, NULL, NULL);
+}
+
+/* As orc_reduce_batched; with w (per column of the row, the reference interface value of the
+   column's knot) also the mean and batch-means per-path variance of the knot's equation estimator
+   score - sum_k g_k w_k: per sub-batch R_b = (sum of scores) - sum_k G_b,k w_k. */
+int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
+                          const double *yt, const double *Y, const double *Z, const double *xi,
+                          const double rect[4], const double glob[4], const int msz[4],
+                          const int *idx, const double *px, const double *py, const double cshape[4],
+                          const double *ipsi, int quirk, const double *w,
+                          int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
+                          double *eq_mean, double *eq_var) {
     int off[5] = {0, msz[0], msz[0] + msz[1], msz[0] + msz[1] + msz[2], msz[0] + msz[1] + msz[2] + msz[3]};
     int64_t ioff[5];
     ioff[0] = 0;
@@ -452,12 +489,12 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
     double *sum = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *sq = (double *)calloc((size_t)nnz + 1, sizeof(double));
     for (int k = 0; k < 6; ++k) cnt[k] = 0;
-    double b = 0.0, bb = 0.0;
+    double b = 0.0, bb = 0.0, rs = 0.0, rq = 0.0;
     int64_t nb = 0;
     /* two-level order: sub-batches of S paths summed into items of P paths, items summed into the knot */
     for (int64_t w0 = 0; w0 < npaths; w0 += P) {
         int64_t w1 = w0 + P < npaths ? w0 + P : npaths;
-        double ib = 0.0, ibb = 0.0;
+        double ib = 0.0, ibb = 0.0, irs = 0.0, irq = 0.0;
         memset(isum, 0, sizeof(double) * (size_t)nnz);
         memset(isq, 0, sizeof(double) * (size_t)nnz);
         for (int64_t p0 = w0; p0 < w1; p0 += S) {
@@ -501,11 +538,18 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
             for (int k = 0; k < nnz; ++k) { isum[k] = isum[k] + Gb[k]; isq[k] = isq[k] + (Gb[k] * Gb[k]) / nbeta; }
             ib = ib + sb;
             ibb = ibb + sbb;
+            if (w) {
+                const double R = sb - orc_wg_sum(Gb, w, nnz);
+                irs = irs + R;
+                irq = irq + (R * R) / nbeta;
+            }
             nb++;
         }
         for (int k = 0; k < nnz; ++k) { sum[k] = sum[k] + isum[k]; sq[k] = sq[k] + isq[k]; }
         b = b + ib;
         bb = bb + ibb;
+        rs = rs + irs;
+        rq = rq + irq;
     }
     double N = (double)npaths;
     for (int k = 0; k < nnz; ++k) {
@@ -514,6 +558,10 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
     }
     *B = b / N;
     *BB = bb / N;
+    if (w && eq_mean && eq_var) {
+        *eq_mean = rs / N;
+        *eq_var = nb > 1 ? (rq - N * *eq_mean * *eq_mean) / (double)(nb - 1) : 0.0;
+    }
     free(slot); free(use); free(Phi); free(Gb); free(isum); free(isq); free(sum); free(sq);
     return nnz;
 }

@@ -114,6 +114,15 @@ int orc_reduce_batched(const orc_problem *pb, int64_t npaths, int64_t S, int64_t
                        const int *idx, const double *px, const double *py, const double cshape[4],
                        const double *ipsi, int quirk,
                        int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6]);
+/* orc_reduce_batched plus the knot equation's mean and batch-means variance with per-column
+   reference values w (the engine's pdds_assemble_eq); w may be NULL */
+int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
+                          const double *yt, const double *Y, const double *Z, const double *xi,
+                          const double rect[4], const double glob[4], const int msz[4],
+                          const int *idx, const double *px, const double *py, const double cshape[4],
+                          const double *ipsi, int quirk, const double *w,
+                          int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
+                          double *eq_mean, double *eq_var);
 
 /* ---- PDDSparseGM::Compute_Solution: BiCGSTAB on (I + G) u = B, zero guess ---- */
 int orc_bicgstab(int n, const int64_t *rowptr, const int *cols, const double *vals, const double *b,

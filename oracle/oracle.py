@@ -66,6 +66,8 @@ def lib():
                _i, _d, _d, _d, _d, _i64]
         L.orc_reduce_faithful.argtypes = red
         L.orc_reduce_batched.argtypes = red[:2] + [C.c_int64, C.c_int64] + red[2:]
+        L.orc_reduce_batched_eq.argtypes = (red[:2] + [C.c_int64, C.c_int64] + red[2:16] + [_d] + red[16:] +
+                                            [_d, _d])
         L.orc_bicgstab.argtypes = [C.c_int, _i64, _i, _d, _d, _d, C.c_double, C.c_int, _d]
         L.orc_philox.argtypes = [_u32, _u32, _u32]
         L.orc_math_eval.argtypes = [C.c_int, _d, _d, C.c_int64]
@@ -198,7 +200,7 @@ def simulate(pb, run, x0, y0, rect, knot, path0, npaths, phase2_out=False):
     return o
 
 
-def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None):
+def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None, w=None):
     msz = np.ascontiguousarray(st["msz"], dtype=np.int32)
     idx = np.ascontiguousarray(st["idx"], dtype=np.int32)
     tot = int(msz.sum())
@@ -217,9 +219,20 @@ def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None):
     args += [ptr(paths["xt"]), ptr(paths["yt"]), ptr(paths["Y"]), ptr(paths["Z"]), ptr(paths["xi"]),
              ptr(np.ascontiguousarray(st["rect"], dtype=np.float64)), ptr(np.ascontiguousarray(glob, dtype=np.float64)),
              ptr(msz, _i), ptr(idx, _i), ptr(np.ascontiguousarray(st["px"])), ptr(np.ascontiguousarray(st["py"])),
-             ptr(cs), ptr(ipsi), quirk, ptr(cols, _i), ptr(G), ptr(Gv), ptr(B), ptr(BB), ptr(cnt, _i64)]
+             ptr(cs), ptr(ipsi), quirk]
+    eq = np.zeros(2)
+    if w is not None:
+        wv = np.zeros(tot + 1)
+        wv[:len(w)] = w
+        args += [ptr(wv)]
+    args += [ptr(cols, _i), ptr(G), ptr(Gv), ptr(B), ptr(BB), ptr(cnt, _i64)]
+    if w is not None:
+        args += [ptr(eq[0:1]), ptr(eq[1:2])]
     nnz = fn(*args)
-    return dict(cols=cols[:nnz].copy(), G=G[:nnz].copy(), Gvar=Gv[:nnz].copy(), B=B[0], BB=BB[0], cnt=cnt)
+    out = dict(cols=cols[:nnz].copy(), G=G[:nnz].copy(), Gvar=Gv[:nnz].copy(), B=B[0], BB=BB[0], cnt=cnt)
+    if w is not None:
+        out["eq_mean"], out["eq_var"] = float(eq[0]), float(eq[1])
+    return out
 
 
 def reduce_faithful(pb, paths, st, glob, cshape, ipsi_list, quirk=1):
@@ -229,6 +242,14 @@ def reduce_faithful(pb, paths, st, glob, cshape, ipsi_list, quirk=1):
 def reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, P=None, quirk=1):
     """Batch-means estimator with the engine's summation order (sub-batches of S in items of P)."""
     return _reduce(lib().orc_reduce_batched, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P)
+
+
+def reduce_batched_eq(pb, paths, st, glob, cshape, ipsi_list, S, P, u_ref, quirk=1):
+    """reduce_batched plus the knot equation's mean and batch-means per-path variance with
+    reference interface values u_ref (indexed by knot), summed in the engine's order."""
+    cols = reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, P, quirk)["cols"]
+    w = np.asarray(u_ref, dtype=np.float64)[cols]
+    return _reduce(lib().orc_reduce_batched_eq, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P, w=w)
 
 
 def bicgstab(rowptr, cols, vals, b, tol=1e-14, maxit=None):

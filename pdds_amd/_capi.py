@@ -28,7 +28,34 @@ class Run(C.Structure):
     _fields_ = [("h", C.c_double), ("beta", C.c_double), ("seed", C.c_uint32), ("stream", C.c_uint32),
                 ("precision", C.c_int32), ("quirk_last_side", C.c_int32), ("max_steps", C.c_int64),
                 ("sub_batch", C.c_int32), ("item_paths", C.c_int32), ("wg_size", C.c_int32),
-                ("grid_wgs", C.c_int32), ("lut_lds", C.c_int32), ("plain", C.c_int32)]
+                ("grid_wgs", C.c_int32), ("lut_lds", C.c_int32), ("plain", C.c_int32),
+                ("paths_per_lane", C.c_int32)]
+
+
+class Adaptive(C.Structure):
+    _fields_ = [("se_target", C.c_double), ("n_min", C.c_int64), ("n_max", C.c_int64), ("n_pilot", C.c_int64),
+                ("max_rounds", C.c_int32), ("tol", C.c_double), ("growth", C.c_double)]
+
+
+class AdaptiveReport(C.Structure):
+    _fields_ = [("rounds", C.c_int32), ("iters", C.c_int32), ("relres", C.c_double), ("paths", C.c_int64),
+                ("path_steps", C.c_int64), ("max_se", C.c_double), ("knots_capped", C.c_int32),
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneConfig(C.Structure):
+    _fields_ = [("sample_paths", C.c_int64), ("max_knots", C.c_int32), ("reps", C.c_int32)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("best_rate", C.c_double), ("base_rate", C.c_double),
+                ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Counters(C.Structure):
@@ -95,6 +122,13 @@ EXPORTS = {
                                     C.c_int64, _d, _d, _d, _d, C.POINTER(Counters)]),
     "pdds_assemble": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _i64, C.c_int64,
                                   _d, _d, _d, _d, C.POINTER(Counters)]),
+    "pdds_assemble_eq": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _i64, C.c_int64,
+                                     _d, _d, _d, _d, _d, _d, _d, C.POINTER(Counters)]),
+    "pdds_adaptive_defaults": (None, [C.POINTER(Adaptive)]),
+    "pdds_solve_adaptive": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run),
+                                        C.POINTER(Adaptive), _d, _i64, _d, C.POINTER(AdaptiveReport)]),
+    "pdds_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), C.POINTER(TuneConfig),
+                                  C.POINTER(Run), C.POINTER(TuneReport)]),
     "pdds_point_eval": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _d, C.c_int32, _d, _d, _i64,
                                     _d, _d, C.POINTER(Counters)]),
     "pdds_solve": (C.c_int32, [C.c_void_p, C.c_int32, _i64, _i32, _d, _d, _d, C.c_double, C.c_int32, _i32, _d]),

@@ -77,6 +77,8 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
         return fail(e, PDDS_ERR_INVALID, "wg_size must be 64, 128 or 256");
     if (r->sub_batch < 0 || r->item_paths < 0 || r->grid_wgs < 0 || r->max_steps < 0)
         return fail(e, PDDS_ERR_INVALID, "negative launch parameter");
+    if (r->paths_per_lane < 0 || r->paths_per_lane > 64)
+        return fail(e, PDDS_ERR_INVALID, "paths_per_lane must be in [0, 64]");
     return PDDS_OK;
 }
 
@@ -138,7 +140,7 @@ void lut_tile_of(const LutView &L, const double rect[4], int &c0, int &r0, int &
 
 pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
                          const std::vector<const DevTables *> &tabs, const std::vector<AsmKnot> &knots,
-                         int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out) {
+                         int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out, const double *W_host) {
     auto host_sd = [&](const AsmKnot &k) -> const StencilDesc & { return tabs[k.table]->host_sten[k.stencil]; };
     PdeParams pp;
     pdds_status st = make_params(e, prob, pp);
@@ -184,6 +186,12 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     double *dG, *dGv;
     if ((st = dbuf(e, B_G, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dG = (double *)p;
     if ((st = dbuf(e, B_GVAR, (size_t)std::max<int64_t>(n_vals, 1) * 8, &p))) return st; dGv = (double *)p;
+    double *dW = nullptr;
+    if (W_host && n_vals > 0) {
+        if ((st = dbuf(e, B_WREF, (size_t)n_vals * 8, &p))) return st;
+        dW = (double *)p;
+        HIPCHK(hipMemcpyAsync(dW, W_host, (size_t)n_vals * 8, hipMemcpyHostToDevice, e->stream));
+    }
     std::vector<TableDesc> tdesc(tabs.size());
     for (size_t t = 0; t < tabs.size(); ++t)
         tdesc[t] = TableDesc{tabs[t]->sten, tabs[t]->entx, tabs[t]->enty, tabs[t]->ents, tabs[t]->e1, tabs[t]->e2,
@@ -288,6 +296,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         for (int i = 0; i < 4; ++i) pa.glob[i] = glob[i];
         pa.h = r->h; pa.sqrth = std::sqrt(r->h); pa.beta = r->beta;
         pa.seed = r->seed; pa.stream = r->stream; pa.max_steps = max_steps; pa.pp = pp;
+        pa.pool_chunk = 64 * (int64_t)(r->paths_per_lane > 0 ? r->paths_per_lane : 4);
         int grid;
         if (persistent) {
             grid = r->grid_wgs > 0 ? r->grid_wgs : e->path_wgs_per_cu * e->n_cu;
@@ -324,6 +333,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         std::memset(&ra, 0, sizeof(ra));
         ra.knots = dk; ra.tables = dtab; ra.items = dit; ra.rec = rec;
         ra.stats = dstats; ra.partials = dpart; ra.ne_cap = ne_cap; ra.nnz_cap = nnz_cap; ra.S = S;
+        ra.wref = dW;
         const int64_t nit = (int64_t)items.size();
         for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
             ra.item_base = base;
@@ -690,12 +700,13 @@ pdds_status pdds_mesh_csr(const pdds_mesh *m, int64_t *row_ptr, int32_t *cols, i
 }
 
 // ------------------------------------------------------------- assembly ----
-pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
-                          const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var, double *B,
-                          double *BB, pdds_counters *ctr) {
-    if (!e) return PDDS_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(e->mu);
-    double t0 = now_ms();
+// Every Monte Carlo knot of mesh m in one batch (the worker loop of PDDSparseGM::Solve,
+// PDDAlgorithms/PDDSparseGM.cpp:795-848); with u_ref also each knot's equation estimator
+// B_k - sum_j G_kj u_ref_j (mean and per-path variance).  Caller holds e->mu.
+static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                                   const int64_t *paths_per_knot, int64_t n_paths, const double *u_ref, double *G,
+                                   double *G_var, double *B, double *BB, double *eq_mean, double *eq_var,
+                                   AsmOut &o) {
     if (!m || !B || !BB) return fail(e, PDDS_ERR_INVALID, "null mesh or output");
     pdds_status st = check_run(e, r);
     if (st) return st;
@@ -714,6 +725,8 @@ pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         if (M.on_boundary[q]) {
             B[q] = host_g(p->pde, M.x0[q], M.y0[q], pp);
             BB[q] = B[q] * B[q];
+            if (eq_mean) eq_mean[q] = B[q];
+            if (eq_var) eq_var[q] = 0.0;
             continue;
         }
         if (n == 0) return fail(e, PDDS_ERR_INVALID, "interior knot with zero paths");
@@ -721,11 +734,233 @@ pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         knots.push_back(k);
         kmap.push_back(q);
     }
-    AsmOut o;
-    st = run_assembly(e, p, r, M.glob, {T}, knots, M.row_ptr.back(), G, G_var, o);
+    std::vector<double> W;
+    if (u_ref) {
+        W.resize(M.cols.size());
+        for (size_t i = 0; i < M.cols.size(); ++i) W[i] = u_ref[M.cols[i]];
+    }
+    st = run_assembly(e, p, r, M.glob, {T}, knots, M.row_ptr.back(), G, G_var, o, u_ref ? W.data() : nullptr);
     if (st) return st;
-    for (size_t i = 0; i < knots.size(); ++i) { B[kmap[i]] = o.kstats[i].B; BB[kmap[i]] = o.kstats[i].BB; }
+    for (size_t i = 0; i < knots.size(); ++i) {
+        B[kmap[i]] = o.kstats[i].B;
+        BB[kmap[i]] = o.kstats[i].BB;
+        if (eq_mean) eq_mean[kmap[i]] = u_ref ? o.kstats[i].Rm : 0.0;
+        if (eq_var) eq_var[kmap[i]] = u_ref ? o.kstats[i].Rv : 0.0;
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                          const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var, double *B,
+                          double *BB, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const double t0 = now_ms();
+    AsmOut o;
+    pdds_status st = assemble_locked(e, m, p, r, paths_per_knot, n_paths, nullptr, G, G_var, B, BB, nullptr,
+                                     nullptr, o);
+    if (st) return st;
     fill_counters(ctr, o, now_ms() - t0);
+    return PDDS_OK;
+}
+
+pdds_status pdds_assemble_eq(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                             const int64_t *paths_per_knot, int64_t n_paths, const double *u_ref, double *G,
+                             double *G_var, double *B, double *BB, double *eq_mean, double *eq_var,
+                             pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const double t0 = now_ms();
+    if (!u_ref || !eq_var) return fail(e, PDDS_ERR_INVALID, "u_ref and eq_var are required");
+    AsmOut o;
+    pdds_status st = assemble_locked(e, m, p, r, paths_per_knot, n_paths, u_ref, G, G_var, B, BB, eq_mean, eq_var, o);
+    if (st) return st;
+    fill_counters(ctr, o, now_ms() - t0);
+    return PDDS_OK;
+}
+
+// ------------------------------------------------- variance-targeted path counts ----
+void pdds_adaptive_defaults(pdds_adaptive *c) {
+    if (!c) return;
+    c->se_target = 1e-3;
+    c->n_min = 1000;
+    c->n_max = 1000000;
+    c->n_pilot = 0;
+    c->max_rounds = 3;
+    c->tol = 1e-10;
+    c->growth = 1.1;
+}
+
+// Pilot assembly and solve, then rounds of: assemble with the current counts and the previous
+// solution as reference, solve, and raise each knot whose equation standard error
+// sqrt(var_k / N_k) is above the target to N_k = growth * var_k / se^2 (within [n_min, n_max]).
+// Counts only grow; every round re-runs paths [0, N_k) of each knot, so the final G, B and u
+// are those of one plain assembly with the returned counts.  The policy stands in for the
+// commented Compute_h_N of the reference (PDDAlgorithms/PDDSparseGM.cpp:4338-4395, N_i =
+// 4 Var_i (1 - rho_i^2) / (0.6667 eps)^2) with the batch-means variance of the knot equation,
+// control variate included, in place of its Var_i (1 - rho_i^2).
+pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
+                                const pdds_adaptive *cfg_in, double *u, int64_t *paths_per_knot, double *se,
+                                pdds_adaptive_report *rep) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const double t0 = now_ms();
+    if (!m || !u || !paths_per_knot) return fail(e, PDDS_ERR_INVALID, "null mesh or output");
+    pdds_adaptive cfg;
+    pdds_adaptive_defaults(&cfg);
+    if (cfg_in) cfg = *cfg_in;
+    if (!(cfg.se_target > 0) || cfg.n_min < 1 || cfg.n_max < cfg.n_min || cfg.max_rounds < 1 || cfg.max_rounds > 64 ||
+        !(cfg.growth >= 1.0))
+        return fail(e, PDDS_ERR_INVALID, "bad adaptive configuration");
+    const int64_t n_pilot = cfg.n_pilot > 0 ? cfg.n_pilot : cfg.n_min;
+    const Mesh &M = m->m;
+    const int n = M.n_knots();
+    const int64_t nnz = M.row_ptr.back();
+    std::vector<double> G(std::max<int64_t>(nnz, 1)), Gv(std::max<int64_t>(nnz, 1)), B(n), BB(n), em(n), ev(n),
+        uref(n);
+    std::vector<int64_t> N(n, n_pilot);
+    pdds_adaptive_report R;
+    std::memset(&R, 0, sizeof(R));
+    const double tol = cfg.tol > 0 ? cfg.tol : 1e-10;
+    int32_t it = 0;
+    double rr = 0.0;
+    auto assemble = [&](const double *ur) -> pdds_status {
+        AsmOut o;
+        pdds_status st = assemble_locked(e, m, p, r, N.data(), 0, ur, G.data(), Gv.data(), B.data(), BB.data(),
+                                         em.data(), ev.data(), o);
+        if (st) return st;
+        for (const KnotStat &k : o.kstats) R.path_steps += (int64_t)k.steps;
+        R.kernel_ms += o.kernel_ms + o.reduce_ms;
+        return PDDS_OK;
+    };
+    auto solve = [&](double *out) -> pdds_status {
+        return solve_system(e, n, M.row_ptr.data(), M.cols.data(), G.data(), B.data(), out, tol, 0, &it, &rr);
+    };
+    // pilot: a solution to weigh the equations with, then their variances at n_pilot paths
+    pdds_status st;
+    if ((st = assemble(nullptr)) || (st = solve(uref.data())) || (st = assemble(uref.data()))) return st;
+    auto target = [&](int q) -> int64_t {
+        const double want = std::ceil(cfg.growth * ev[q] / (cfg.se_target * cfg.se_target));
+        const double c = std::min(std::max(want, (double)cfg.n_min), (double)cfg.n_max);
+        return (int64_t)c;
+    };
+    for (int q = 0; q < n; ++q)
+        if (!M.on_boundary[q]) N[q] = std::max(N[q], target(q));
+    for (int round = 0; round < cfg.max_rounds; ++round) {
+        if ((st = assemble(uref.data())) || (st = solve(u))) return st;
+        R.rounds = round + 1;
+        std::vector<int> raise;
+        for (int q = 0; q < n; ++q) {
+            if (M.on_boundary[q]) continue;
+            const double sek = std::sqrt(std::max(ev[q], 0.0) / (double)N[q]);
+            if (sek > cfg.se_target && N[q] < cfg.n_max) raise.push_back(q);
+        }
+        // the counts returned are those of the last assembly
+        if (raise.empty() || round + 1 == cfg.max_rounds) break;
+        for (int q : raise) N[q] = std::max(N[q] + 1, target(q));
+        std::copy(u, u + n, uref.begin());
+    }
+    R.paths = 0;
+    R.max_se = 0.0;
+    R.knots_capped = 0;
+    for (int q = 0; q < n; ++q) {
+        paths_per_knot[q] = M.on_boundary[q] ? 0 : N[q];
+        const double sek = M.on_boundary[q] ? 0.0 : std::sqrt(std::max(ev[q], 0.0) / (double)N[q]);
+        if (se) se[q] = sek;
+        if (M.on_boundary[q]) continue;
+        R.paths += N[q];
+        R.max_se = std::max(R.max_se, sek);
+        if (sek > cfg.se_target && N[q] >= cfg.n_max) R.knots_capped++;
+    }
+    R.iters = it;
+    R.relres = rr;
+    R.total_ms = now_ms() - t0;
+    if (rep) *rep = R;
+    return PDDS_OK;
+}
+
+// ------------------------------------------------------------- autotuner ----
+pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *base,
+                          const pdds_tune_config *cfg_in, pdds_run *best, pdds_tune_report *rep) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const double t0 = now_ms();
+    if (!m || !p || !best) return fail(e, PDDS_ERR_INVALID, "null mesh, problem or output");
+    pdds_status st = check_run(e, base);
+    if (st) return st;
+    pdds_tune_config cfg{4096, 256, 2};
+    if (cfg_in) {
+        if (cfg_in->sample_paths > 0) cfg.sample_paths = cfg_in->sample_paths;
+        if (cfg_in->max_knots > 0) cfg.max_knots = cfg_in->max_knots;
+        if (cfg_in->reps > 0) cfg.reps = cfg_in->reps;
+    }
+    const Mesh &M = m->m;
+    if (M.any_singular) return fail(e, PDDS_ERR_SINGULAR, "mesh has a singular Psi");
+    PdeParams pp;
+    if ((st = make_params(e, p, pp))) return st;
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    DevTables *T;
+    if ((st = mesh_dev_tables(e, m, base->quirk_last_side, &T))) return st;
+    std::vector<int> mc;
+    for (int q = 0; q < M.n_knots(); ++q)
+        if (!M.on_boundary[q]) mc.push_back(q);
+    if (mc.empty()) return fail(e, PDDS_ERR_INVALID, "mesh has no Monte Carlo knot");
+    const size_t stride = (mc.size() + cfg.max_knots - 1) / (size_t)cfg.max_knots;
+    std::vector<AsmKnot> knots;
+    for (size_t i = 0; i < mc.size(); i += stride) {
+        const int q = mc[i];
+        knots.push_back(AsmKnot{M.x0[q], M.y0[q], (uint32_t)q, M.stencil_of[q], cfg.sample_paths, M.row_ptr[q]});
+    }
+    // candidate launch shapes (none of them changes a result bit)
+    std::vector<pdds_run> cands;
+    auto add = [&](int lds, int wg, int grid, int ppl) {
+        pdds_run r = *base;
+        r.plain = 0;
+        r.lut_lds = lds;
+        r.wg_size = wg;
+        r.grid_wgs = grid;
+        r.paths_per_lane = ppl;
+        cands.push_back(r);
+    };
+    {
+        pdds_run r = *base;
+        r.plain = 0;
+        cands.push_back(r);
+    }
+    const bool can_tile = p->pde == PDDS_PDE_VARCOEF_LUT && pp.lut.nf == 5;
+    if (can_tile)
+        for (int g : {1, 2})
+            for (int ppl : {1, 2, 4, 8}) add(1, 0, g * e->n_cu, ppl);
+    for (int wg : {64, 128, 256})
+        for (int g : {1, 2, 4, 8})
+            for (int ppl : {1, 4, 16}) add(0, wg, g * e->n_cu * 256 / wg, ppl);
+    double best_rate = -1.0, base_rate = 0.0;
+    size_t best_i = 0;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        double tmin = 1e300;
+        int64_t steps = 0;
+        for (int rep_i = 0; rep_i <= cfg.reps; ++rep_i) {
+            AsmOut o;
+            const double ts = now_ms();
+            if ((st = run_assembly(e, p, &cands[i], M.glob, {T}, knots, M.row_ptr.back(), nullptr, nullptr, o)))
+                return st;
+            const double dt = now_ms() - ts;
+            if (rep_i == 0) continue;   // warm-up
+            tmin = std::min(tmin, dt);
+            steps = 0;
+            for (const KnotStat &k : o.kstats) steps += (int64_t)k.steps;
+        }
+        const double rate = (double)steps / (tmin * 1e-3);
+        if (i == 0) base_rate = rate;
+        if (rate > best_rate) { best_rate = rate; best_i = i; }
+    }
+    *best = cands[best_i];
+    if (rep) {
+        rep->candidates = (int32_t)cands.size();
+        rep->best_rate = best_rate;
+        rep->base_rate = base_rate;
+        rep->total_ms = now_ms() - t0;
+    }
     return PDDS_OK;
 }
 

@@ -76,6 +76,7 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
+    __shared__ double s_wsum[(kReduceWG / 64) * kNBG];
     __shared__ int s_lcnt[kNBG * 4];
     __shared__ unsigned long long s_steps;
     __shared__ unsigned int s_cnt[8];
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
     eoff[0] = 0;
     for (int q = 0; q < 4; ++q) eoff[q + 1] = eoff[q] + sd.msz[q];
     for (int c = tid; c < nnz; c += WG) { accS[c] = 0.0; accQ[c] = 0.0; }
-    double Bs = 0.0, BBs = 0.0;
+    double Bs = 0.0, BBs = 0.0, Rs = 0.0, RQs = 0.0;
     for (int g = 0; g < nsb; g += kNBG) {
         const int nb = (nsb - g) < kNBG ? (nsb - g) : kNBG;
         const int q0 = g * S;                                   // first path of the group
@@ -196,6 +197,9 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
             }
         }
         __syncthreads();
+        double tp[kNBG];   // this thread's columns' sum of G_b,c u_ref (equation residual)
+#pragma unroll
+        for (int b = 0; b < kNBG; ++b) tp[b] = 0.0;
         for (int c = tid; c < nnz; c += WG) {
             double G[kNBG];
 #pragma unroll
@@ -249,9 +253,35 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
             }
             accS[c] = sum;
             accQ[c] = q2;
+            if (a.wref) {
+                const double wc = a.wref[kd.out_off + c];
+#pragma unroll
+                for (int b = 0; b < kNBG; ++b) tp[b] = tp[b] + G[b] * wc;
+            }
         }
-        if (tid == 0)
+        if (a.wref) {
+            // sum over the columns: a butterfly over each wave, then the waves in order
+#pragma unroll
+            for (int b = 0; b < kNBG; ++b) {
+                double v = tp[b];
+                for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+                if (lane == 0) s_wsum[wave * kNBG + b] = v;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
             for (int b = 0; b < nb; ++b) { Bs = Bs + s_bsb[b]; BBs = BBs + s_bbsb[b]; }
+            if (a.wref)
+                for (int b = 0; b < nb; ++b) {
+                    double T = 0.0;
+                    for (int w = 0; w < nwaves; ++w) T = T + s_wsum[w * kNBG + b];
+                    const int p0 = (g + b) * S;
+                    const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
+                    const double R = s_bsb[b] - T;
+                    Rs = Rs + R;
+                    RQs = RQs + (R * R) / (double)(p1 - p0);
+                }
+        }
         __syncthreads();
     }
     for (int c = tid; c < nnz; c += WG) {
@@ -262,6 +292,8 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
         ItemStat st;
         st.Bs = Bs;
         st.BBs = BBs;
+        st.Rs = Rs;
+        st.RQs = RQs;
         st.steps = s_steps;
         for (int q = 0; q < 6; ++q) st.cnt[q] = s_cnt[q];
         st.capped = s_cnt[6];
@@ -297,7 +329,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
         f.Gvar[kd.out_off + c] = nb > 1 ? (q2 - N * G * G) / (double)(nb - 1) : 0.0;
     }
     if (threadIdx.x == 0) {
-        double Bs = 0.0, BBs = 0.0;
+        double Bs = 0.0, BBs = 0.0, Rs = 0.0, RQs = 0.0;
         KnotStat ks;
         ks.steps = 0;
         ks.capped = 0;
@@ -306,12 +338,16 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
             const ItemStat st = f.stats[w];
             Bs = Bs + st.Bs;
             BBs = BBs + st.BBs;
+            Rs = Rs + st.Rs;
+            RQs = RQs + st.RQs;
             ks.steps += st.steps;
             ks.capped += st.capped;
             for (int q = 0; q < 6; ++q) ks.cnt[q] += st.cnt[q];
         }
         ks.B = Bs / N;
         ks.BB = BBs / N;
+        ks.Rm = Rs / N;
+        ks.Rv = nb > 1 ? (RQs - N * ks.Rm * ks.Rm) / (double)(nb - 1) : 0.0;
         f.kstats[k] = ks;
     }
 }

@@ -74,7 +74,7 @@ struct pdds_mesh {
 
 enum BufSlot {
     B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_RECX, B_RECY,
-    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_PATCH, B_TABLES, B_G, B_GVAR, B_KSTAT, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
+    B_RECYY, B_RECS, B_RECSIDE, B_RECSTEPS, B_COUNTER, B_PATCH, B_TABLES, B_G, B_GVAR, B_KSTAT, B_WREF, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
     B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
 };
 
@@ -131,9 +131,12 @@ pdds_status build_tables(pdds_engine *e, const std::vector<StencilDesc> &sten, c
                          const std::vector<double> &enty, const std::vector<int8_t> &ents,
                          const std::vector<int32_t> &e1, const std::vector<int32_t> &e2,
                          const std::vector<double> &ipsi, DevTables &T);
+// W_host (optional, n_vals values): per CSR value the reference interface value of its column's
+// knot; the knots' equation means and variances then land in out.kstats[].Rm / Rv
 pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_run *r, const double glob[4],
                          const std::vector<const DevTables *> &tabs, const std::vector<AsmKnot> &knots,
-                         int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out);
+                         int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out,
+                         const double *W_host = nullptr);
 void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::vector<double> &entx,
                  std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<int32_t> &e1,
                  std::vector<int32_t> &e2, std::vector<double> &ipsi);

@@ -276,8 +276,8 @@ struct Walker {
     }
 };
 
-// Wave-private pool of consecutive paths [next, end) taken kPoolChunk at a time from one
-// queue (one atomic per kPoolChunk paths).  With it, the knot of the pool's first path and
+// Wave-private pool of consecutive paths [next, end) taken a.pool_chunk at a time from one
+// queue (one atomic per a.pool_chunk paths: 64 lanes x the run's paths per lane).  With it, the knot of the pool's first path and
 // where that knot ends, found once per chunk with wave-uniform values; a lane whose path
 // lies past that knot searches for its own.
 struct WavePool {
@@ -289,12 +289,12 @@ struct WavePool {
     __device__ __forceinline__ bool take(const PathKernelArgs &a, unsigned long long *counter, int64_t qb,
                                          int64_t qe, int lane) {
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(counter, (unsigned long long)kPoolChunk);
+        if (lane == 0) base = atomicAdd(counter, (unsigned long long)a.pool_chunk);
         base = uniform_u64(base);
         const int64_t s0 = qb + (int64_t)base;
         if (s0 >= qe) return false;
         next = s0;
-        end = s0 + kPoolChunk < qe ? s0 + kPoolChunk : qe;
+        end = s0 + a.pool_chunk < qe ? s0 + a.pool_chunk : qe;
         ck = find_knot(a.knots, a.n_knots, s0);
         ck_end = ck + 1 < a.n_knots ? a.knots[ck + 1].path_off : a.total;
         return true;

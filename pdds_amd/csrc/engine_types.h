@@ -18,7 +18,6 @@ inline size_t reduce_lds_bytes(int ne_cap, int nnz_cap, int S) {
     return (b + 15) & ~(size_t)15;
 }
 constexpr int32_t kCappedBit = 1 << 30;    // record flag: path stopped by the step cap
-constexpr int64_t kPoolChunk = 256;        // paths a wave takes from a path queue at once
 constexpr int kMaxQueues = 8;              // path queues: one per XCD of an MI355X (8 XCDs x 32 CUs)
 
 // One Monte Carlo knot of an assembly batch.
@@ -68,6 +67,7 @@ struct ItemDesc {
 
 struct ItemStat {
     double Bs, BBs;
+    double Rs, RQs;         // equation residual sums (ReduceArgs::wref set): sum R_b, sum R_b^2 / n_b
     unsigned long long steps;
     uint32_t cnt[6];
     uint32_t capped;
@@ -76,6 +76,7 @@ struct ItemStat {
 
 struct KnotStat {
     double B, BB;
+    double Rm, Rv;          // mean and per-path variance of the knot's equation estimator
     unsigned long long steps;
     unsigned long long capped;
     unsigned long long cnt[6];
@@ -108,6 +109,7 @@ struct PathKernelArgs {
     // dispatched to) and moves to the next queue when its own is drained
     unsigned long long *counter;
     int32_t n_queues;
+    int64_t pool_chunk;     // paths a wave takes from a queue at once (64 x paths per lane)
     // LDS-tiled launches: one queue per patch (counter[p] counts paths handed out of patch p)
     const struct PatchDesc *patches;
     int32_t n_patches;
@@ -127,6 +129,9 @@ struct ReduceArgs {
     Records rec;
     ItemStat *stats;
     double *partials;
+    // optional: per CSR value (out_off + c), the reference interface value u_ref of the column's
+    // knot; the reduction then also sums R_b = B_b - sum_c G_b,c u_ref per sub-batch b
+    const double *wref;
     int32_t ne_cap, nnz_cap;
     int32_t S;
 };

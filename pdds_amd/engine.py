@@ -136,6 +136,48 @@ class Engine:
                                         n_paths, ptr(G), ptr(Gv), ptr(B), ptr(BB), C.byref(ctr)), "assemble")
         return {"G": G[:nnz], "Gvar": Gv[:nnz], "B": B, "BB": BB, "counters": ctr.as_dict()}
 
+    def assemble_eq(self, mesh, problem, run, n_paths, u_ref, paths_per_knot=None):
+        """pdds_assemble_eq: assemble() plus each knot equation's mean and per-path variance
+        with reference interface values u_ref."""
+        nnz = mesh.nnz
+        G = np.zeros(max(nnz, 1))
+        Gv = np.zeros(max(nnz, 1))
+        B, BB, em, ev = (np.zeros(mesh.n_knots) for _ in range(4))
+        ppk = None if paths_per_knot is None else np.ascontiguousarray(paths_per_knot, dtype=np.int64)
+        ur = f64(u_ref)
+        if ur.shape != (mesh.n_knots,):
+            raise ValueError("u_ref needs one value per knot")
+        ctr = A.Counters()
+        self._chk(A.lib().pdds_assemble_eq(self.h, mesh.h, C.byref(problem._c()), C.byref(run), ptr(ppk, A._i64),
+                                           n_paths, ptr(ur), ptr(G), ptr(Gv), ptr(B), ptr(BB), ptr(em), ptr(ev),
+                                           C.byref(ctr)), "assemble_eq")
+        return {"G": G[:nnz], "Gvar": Gv[:nnz], "B": B, "BB": BB, "eq_mean": em, "eq_var": ev,
+                "counters": ctr.as_dict()}
+
+    def solve_adaptive(self, mesh, problem, run, **cfg):
+        """pdds_solve_adaptive: variance-targeted per-knot path counts (keys of pdds_adaptive)."""
+        c = A.Adaptive()
+        A.lib().pdds_adaptive_defaults(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        n = mesh.n_knots
+        u = np.zeros(n)
+        N = np.zeros(n, dtype=np.int64)
+        se = np.zeros(n)
+        rep = A.AdaptiveReport()
+        self._chk(A.lib().pdds_solve_adaptive(self.h, mesh.h, C.byref(problem._c()), C.byref(run), C.byref(c),
+                                              ptr(u), ptr(N, A._i64), ptr(se), C.byref(rep)), "solve_adaptive")
+        return {"u": u, "paths_per_knot": N, "se": se, "report": rep.as_dict()}
+
+    def autotune(self, mesh, problem, run, sample_paths=0, max_knots=0, reps=0):
+        """pdds_autotune: returns (tuned run, report)."""
+        cfg = A.TuneConfig(sample_paths, max_knots, reps)
+        best = A.Run()
+        rep = A.TuneReport()
+        self._chk(A.lib().pdds_autotune(self.h, mesh.h, C.byref(problem._c()), C.byref(run), C.byref(cfg),
+                                        C.byref(best), C.byref(rep)), "autotune")
+        return best, rep.as_dict()
+
     def point_eval(self, problem, run, glob, x, y, n_paths):
         x = f64(x)
         y = f64(y)

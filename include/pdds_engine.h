@@ -201,13 +201,14 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
                                 pdds_adaptive_report *rep);
 
 /* ---- launch autotuner (no counterpart in the reference) ----
-   Times trial assemblies of a sample of the mesh's knots (every k-th Monte Carlo knot, at most
-   max_knots of them, sample_paths paths each) over the launch shapes that do not change results
+   Times trial assemblies of a sample of the mesh's knots (a contiguous run of at most max_knots
+   Monte Carlo knots from the middle of the mesh, sample_paths paths each, so that the sample keeps
+   the mesh's stencil and LUT-tile structure) over the launch shapes that do not change results
    -- LDS tiling on or off, work-group size, work-groups per CU, paths per lane -- and writes into
    *best the base run with the fastest shape (path-steps per second of the whole assembly). */
 typedef struct {
-    int64_t sample_paths;   /* paths per sampled knot (0: 4096) */
-    int32_t max_knots;      /* sampled knots (0: 256) */
+    int64_t sample_paths;   /* paths per sampled knot (0: 16384) */
+    int32_t max_knots;      /* sampled knots (0: 512) */
     int32_t reps;           /* timed repetitions per candidate, after one warm-up (0: 2) */
 } pdds_tune_config;
 typedef struct {

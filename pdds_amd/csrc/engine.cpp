@@ -888,7 +888,7 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
     if (!m || !p || !best) return fail(e, PDDS_ERR_INVALID, "null mesh, problem or output");
     pdds_status st = check_run(e, base);
     if (st) return st;
-    pdds_tune_config cfg{4096, 256, 2};
+    pdds_tune_config cfg{16384, 512, 2};
     if (cfg_in) {
         if (cfg_in->sample_paths > 0) cfg.sample_paths = cfg_in->sample_paths;
         if (cfg_in->max_knots > 0) cfg.max_knots = cfg_in->max_knots;
@@ -905,9 +905,12 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
     for (int q = 0; q < M.n_knots(); ++q)
         if (!M.on_boundary[q]) mc.push_back(q);
     if (mc.empty()) return fail(e, PDDS_ERR_INVALID, "mesh has no Monte Carlo knot");
-    const size_t stride = (mc.size() + cfg.max_knots - 1) / (size_t)cfg.max_knots;
+    // a contiguous run of knots from the middle of the mesh: neighbouring knots share stencils
+    // and LUT tiles, so the sample has the patch structure of the whole mesh
+    const size_t nk = std::min(mc.size(), (size_t)cfg.max_knots);
+    const size_t k0 = (mc.size() - nk) / 2;
     std::vector<AsmKnot> knots;
-    for (size_t i = 0; i < mc.size(); i += stride) {
+    for (size_t i = k0; i < k0 + nk; ++i) {
         const int q = mc[i];
         knots.push_back(AsmKnot{M.x0[q], M.y0[q], (uint32_t)q, M.stencil_of[q], cfg.sample_paths, M.row_ptr[q]});
     }

@@ -105,6 +105,22 @@ pdds_status pdds_lut_eval(pdds_engine *e, const pdds_lut *l, int32_t field, int6
 /* ---- mesh: interfaces, knots, stencils, CSR pattern, Psi^-1 ---- */
 pdds_status pdds_mesh_create(double sw_x, double sw_y, double ne_x, double ne_y, int32_t n_sub,
                              int32_t n_node, const double fac[4], pdds_mesh **out);
+/* Disc mesh (CircleMesh layout; the reference's Meshes/Circle*.hpp are empty, so the layout is
+   this engine's): a disc of `radius` at the origin cut into n_rings rings (radii radius*i/n_rings)
+   x n_sectors sectors; every arc and ray-segment interface carries kmin..kmax equispaced interior
+   knots (count from a hash of seed and the interface), with a knot at every crossing of an inner
+   circle and a ray and a boundary knot where each ray meets the rim.  A knot's stencil is the
+   annular sector of the subdomains around its interface, with sides outer arc (0), inner arc (1),
+   end ray (2) and start ray (3); each side's Gaussian shape starts at shape x 3.5 x its mean knot
+   spacing and shrinks until the side's Psi has cond below 1e11. */
+pdds_status pdds_mesh_create_disc(double radius, int32_t n_rings, int32_t n_sectors, uint32_t seed, int32_t kmin,
+                                  int32_t kmax, double shape, pdds_mesh **out);
+/* stencil stopping domain of a knot: *geom 0 rectangle (domain[0..3] = x0, y0, x1, y1) or 1 annular
+   sector (domain = r0, r1, cos a, sin a, cos b, sin b); zeros for a knot without a stencil */
+pdds_status pdds_mesh_geometry(const pdds_mesh *m, int32_t knot, int32_t *geom, double domain[6]);
+/* Gaussian RBF shape c of each stencil side of a knot (Compute_ipsi's fac[kind] on rectangular
+   meshes, PDDAlgorithms/PDDSparseGM.cpp:816) */
+pdds_status pdds_mesh_shape(const pdds_mesh *m, int32_t knot, double c[4]);
 void pdds_mesh_destroy(pdds_mesh *m);
 int32_t pdds_mesh_n_knots(const pdds_mesh *m);
 int32_t pdds_mesh_n_stencils(const pdds_mesh *m);      /* distinct stencils */

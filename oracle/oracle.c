@@ -97,6 +97,47 @@ double orc_rect_dist(const double p[4], double x, double y, double *ex, double *
     return dy - hy;
 }
 
+/* Signed distance to the annular sector g = {r0, r1, cos a, sin a, cos b, sin b} of a disc mesh
+   (the engine's own CircleMesh layout; the reference's Meshes/Circle*.hpp files are empty):
+   the largest of the four constraint violations r - r1, r0 - r (none when r0 = 0), and the
+   signed distances beyond the rays a and b.  Projection onto that side, clamped to it, and its
+   outward normal; kind 0 outer arc, 1 inner arc, 2 ray b, 3 ray a. */
+double orc_sector_dist(const double g[6], double x, double y, double *ex, double *ey, double *nx, double *ny,
+                       int *kind) {
+    const double r0 = g[0], r1 = g[1], ca = g[2], sa = g[3], cb = g[4], sb = g[5];
+    const double r = sqrt(x * x + y * y);
+    const double sA = ca * y - sa * x, sB = x * sb - y * cb;
+    double d = r - r1;
+    int k = 0;
+    const double din = r0 > 0.0 ? r0 - r : -(r + r1);
+    if (din > d) { d = din; k = 1; }
+    if (-sB > d) { d = -sB; k = 2; }
+    if (-sA > d) { d = -sA; k = 3; }
+    if (k <= 1) {
+        double ux = x / r, uy = y / r;
+        if (sA < 0.0) { ux = ca; uy = sa; }
+        else if (sB < 0.0) { ux = cb; uy = sb; }
+        const double rr = k == 0 ? r1 : r0;
+        *ex = rr * ux; *ey = rr * uy;
+        *nx = k == 0 ? ux : -ux; *ny = k == 0 ? uy : -uy;
+    } else {
+        const double dx = k == 3 ? ca : cb, dy = k == 3 ? sa : sb;
+        double t = x * dx + y * dy;
+        t = t < r0 ? r0 : (t > r1 ? r1 : t);
+        *ex = t * dx; *ey = t * dy;
+        *nx = k == 3 ? sa : -sb;
+        *ny = k == 3 ? -ca : cb;
+    }
+    if (kind) *kind = k;
+    return d;
+}
+
+/* distance to a stencil's stopping domain: geom 0 rectangle dom[4], geom 1 sector dom[6] */
+static inline double orc_domain_dist(int geom, const double *dom, double x, double y, double *ex, double *ey,
+                                     double *nx, double *ny) {
+    return geom ? orc_sector_dist(dom, x, y, ex, ey, nx, ny, NULL) : orc_rect_dist(dom, x, y, ex, ey, nx, ny);
+}
+
 /* ------------------------------------------------------------------------ */
 /* PDE coefficients (reference convention, BVPs/BVP.hpp:89-137)              */
 /* ------------------------------------------------------------------------ */
@@ -252,10 +293,10 @@ static inline void orc_step(orc_state *s, double h, double sqrth, double n1, dou
 /* Inside() with the Gobet shift (Integrators/FKACIntegrator.hpp:168-186): returns 1 while the
    path continues, i.e. dist < beta * |N^T sigma(X)| * sqrt(h).  Evaluates the coefficients at
    the new X (sigma is needed here, the rest by the next step). */
-static inline int orc_inside(const orc_problem *pb, orc_state *s, const double rect[4], double beta,
+static inline int orc_inside(const orc_problem *pb, orc_state *s, int geom, const double *dom, double beta,
                              double sqrth, double *ex, double *ey) {
     double nx, ny;
-    double d = orc_rect_dist(rect, s->x, s->y, ex, ey, &nx, &ny);
+    double d = orc_domain_dist(geom, dom, s->x, s->y, ex, ey, &nx, &ny);
     orc_coeffs(pb, s);
     double a = nx * s->sig[0] + ny * s->sig[2];
     double b = nx * s->sig[1] + ny * s->sig[3];
@@ -263,10 +304,10 @@ static inline int orc_inside(const orc_problem *pb, orc_state *s, const double r
     return d < beta * nrm * sqrth;
 }
 
-void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y0, const double rect[4],
-                  uint32_t knot, int64_t path0, int64_t npaths,
-                  double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
-                  double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2) {
+static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double x0, double y0, int geom,
+                              const double *rect, uint32_t knot, int64_t path0, int64_t npaths,
+                              double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
+                              double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2) {
     const double h = run->h, sqrth = sqrt(h);
     const int64_t cap = run->max_steps > 0 ? run->max_steps : INT64_MAX;
 #ifdef _OPENMP
@@ -286,7 +327,7 @@ void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y
             orc_normal_pair(run, knot, path, st, &n1, &n2);
             orc_step(&s, h, sqrth, n1, n2);
             ++st;
-            in = orc_inside(pb, &s, rect, run->beta, sqrth, &ex, &ey);
+            in = orc_inside(pb, &s, geom, rect, run->beta, sqrth, &ex, &ey);
         } while (in && st < cap);
         if (xt) xt[n] = ex;
         if (yt) yt[n] = ey;
@@ -302,7 +343,7 @@ void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y
                 orc_normal_pair(run, knot, path, st + st2, &n1, &n2);
                 orc_step(&s, h, sqrth, n1, n2);
                 ++st2;
-                in = orc_inside(pb, &s, rect, 0.0, sqrth, &ex, &ey);
+                in = orc_inside(pb, &s, geom, rect, 0.0, sqrth, &ex, &ey);
             } while (in && st2 < cap);
             if (xt2) xt2[n] = ex;
             if (yt2) yt2[n] = ey;
@@ -312,6 +353,21 @@ void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y
             if (steps2) steps2[n] = st2;
         }
     }
+}
+
+void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y0, const double rect[4],
+                  uint32_t knot, int64_t path0, int64_t npaths,
+                  double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
+                  double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2) {
+    orc_simulate_impl(pb, run, x0, y0, 0, rect, knot, path0, npaths, xt, yt, Y, Z, xi, steps, xt2, yt2, Y2, Z2,
+                      xi2, steps2);
+}
+
+void orc_simulate_geom(const orc_problem *pb, const orc_run *run, double x0, double y0, int geom,
+                       const double *dom, uint32_t knot, int64_t path0, int64_t npaths,
+                       double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps) {
+    orc_simulate_impl(pb, run, x0, y0, geom, dom, knot, path0, npaths, xt, yt, Y, Z, xi, steps, NULL, NULL, NULL,
+                      NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -329,6 +385,19 @@ static inline int orc_classify(const double rect[4], const double glob[4], doubl
     if (orc_same(y, rect[3])) return 0;
     if (orc_same(x, rect[0])) return 3;
     if (orc_same(x, rect[2])) return 2;
+    return -1;
+}
+
+/* Exit side on a sector stencil dom of a disc of radius R (glob[2]): global if on the disc
+   boundary, then inner arc, outer arc, ray a, ray b (the S, N, W, E priority of the rectangle). */
+static inline int orc_classify_geom(int geom, const double *dom, const double glob[4], double x, double y) {
+    if (!geom) return orc_classify(dom, glob, x, y);
+    const double r = sqrt(x * x + y * y);
+    if (r >= glob[2] - 1e-8) return 4;
+    if (dom[0] > 0.0 && orc_same(r, dom[0])) return 1;
+    if (orc_same(r, dom[1])) return 0;
+    if (fabs(dom[2] * y - dom[3] * x) < 1e-8) return 3;
+    if (fabs(x * dom[5] - y * dom[4]) < 1e-8) return 2;
     return -1;
 }
 
@@ -467,6 +536,19 @@ int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int6
                           const double *ipsi, int quirk, const double *w,
                           int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                           double *eq_mean, double *eq_var) {
+    return orc_reduce_batched_geom(pb, npaths, S, P, xt, yt, Y, Z, xi, 0, rect, glob, msz, idx, px, py, cshape,
+                                   ipsi, quirk, w, cols, G, Gvar, B, BB, cnt, eq_mean, eq_var);
+}
+
+/* As orc_reduce_batched_eq, with the exit sides classified on stopping domain dom of geometry
+   geom (0 rectangle, 1 annular sector of a disc of radius glob[2]). */
+int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
+                            const double *yt, const double *Y, const double *Z, const double *xi, int geom,
+                            const double *rect, const double glob[4], const int msz[4],
+                            const int *idx, const double *px, const double *py, const double cshape[4],
+                            const double *ipsi, int quirk, const double *w,
+                            int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
+                            double *eq_mean, double *eq_var) {
     int off[5] = {0, msz[0], msz[0] + msz[1], msz[0] + msz[1] + msz[2], msz[0] + msz[1] + msz[2] + msz[3]};
     int64_t ioff[5];
     ioff[0] = 0;
@@ -502,7 +584,7 @@ int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int6
             memset(Phi, 0, sizeof(double) * (size_t)tot);
             double sb = 0.0, sbb = 0.0;
             for (int64_t n = p0; n < p1; ++n) {
-                int side = orc_classify(rect, glob, xt[n], yt[n]);
+                int side = orc_classify_geom(geom, rect, glob, xt[n], yt[n]);
                 double sc;
                 if (side == 4) {
                     double sig[4], bv[2], c, f, grad[2], g, u;

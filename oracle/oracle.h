@@ -64,6 +64,8 @@ void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_math_eval(int fn, const double *x, double *y, int64_t n);   /* 0 exp,1 log,2 sin,3 cos,4 tanh,5 sinpi,6 cospi,7 expm1 */
 void orc_normals(const orc_run *run, uint32_t knot, uint32_t path, int64_t step0, int64_t nsteps, double *n1, double *n2);
 double orc_rect_dist(const double p[4], double x, double y, double *ex, double *ey, double *nx, double *ny);
+double orc_sector_dist(const double g[6], double x, double y, double *ex, double *ey, double *nx, double *ny,
+                       int *kind);
 void orc_pde_eval(const orc_problem *pb, double x, double y, double sig[4], double b[2],
                   double *c, double *f, double grad[2], double *g, double *u);
 double orc_lut_eval(const orc_problem *pb, int field, double x, double y);
@@ -92,6 +94,10 @@ void orc_simulate(const orc_problem *pb, const orc_run *run, double x0, double y
                   uint32_t knot, int64_t path0, int64_t npaths,
                   double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
                   double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2);
+/* phase 1 on a stopping domain of geometry geom: 0 rectangle dom[4], 1 annular sector dom[6] */
+void orc_simulate_geom(const orc_problem *pb, const orc_run *run, double x0, double y0, int geom,
+                       const double *dom, uint32_t knot, int64_t path0, int64_t npaths,
+                       double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps);
 
 /* ---- reductions: Reduce_Analytic + Stencil::G_update + G_return_withrep ----
    Inputs: per-path records, stencil (sizes, concatenated positions/indices, per-side shape c,
@@ -123,6 +129,14 @@ int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int6
                           const double *ipsi, int quirk, const double *w,
                           int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                           double *eq_mean, double *eq_var);
+/* orc_reduce_batched_eq on a stopping domain of geometry geom (disc radius in glob[2]) */
+int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
+                            const double *yt, const double *Y, const double *Z, const double *xi, int geom,
+                            const double *rect, const double glob[4], const int msz[4],
+                            const int *idx, const double *px, const double *py, const double cshape[4],
+                            const double *ipsi, int quirk, const double *w,
+                            int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
+                            double *eq_mean, double *eq_var);
 
 /* ---- PDDSparseGM::Compute_Solution: BiCGSTAB on (I + G) u = B, zero guess ---- */
 int orc_bicgstab(int n, const int64_t *rowptr, const int *cols, const double *vals, const double *b,

@@ -57,6 +57,10 @@ def lib():
         L.orc_stencil_ipsi.argtypes = [C.c_int, _d, _d, C.c_double, _d, _d]
         L.orc_rect_dist.restype = C.c_double
         L.orc_rect_dist.argtypes = [_d, C.c_double, C.c_double, _d, _d, _d, _d]
+        L.orc_sector_dist.restype = C.c_double
+        L.orc_sector_dist.argtypes = [_d, C.c_double, C.c_double, _d, _d, _d, _d, _i]
+        L.orc_simulate_geom.argtypes = [C.POINTER(Problem), C.POINTER(Run), C.c_double, C.c_double, C.c_int, _d,
+                                        C.c_uint32, C.c_int64, C.c_int64] + [_d] * 5 + [_i64]
         L.orc_pde_eval.argtypes = [C.POINTER(Problem), C.c_double, C.c_double, _d, _d, _d, _d, _d, _d, _d]
         L.orc_lut_eval.restype = C.c_double
         L.orc_lut_eval.argtypes = [C.POINTER(Problem), C.c_int, C.c_double, C.c_double]
@@ -68,6 +72,8 @@ def lib():
         L.orc_reduce_batched.argtypes = red[:2] + [C.c_int64, C.c_int64] + red[2:]
         L.orc_reduce_batched_eq.argtypes = (red[:2] + [C.c_int64, C.c_int64] + red[2:16] + [_d] + red[16:] +
                                             [_d, _d])
+        L.orc_reduce_batched_geom.argtypes = (red[:2] + [C.c_int64, C.c_int64] + red[2:7] + [C.c_int] + red[7:16] +
+                                              [_d] + red[16:] + [_d, _d])
         L.orc_bicgstab.argtypes = [C.c_int, _i64, _i, _d, _d, _d, C.c_double, C.c_int, _d]
         L.orc_philox.argtypes = [_u32, _u32, _u32]
         L.orc_math_eval.argtypes = [C.c_int, _d, _d, C.c_int64]
@@ -104,6 +110,15 @@ def rect_dist(p, x, y):
     o = np.zeros(4)
     d = lib().orc_rect_dist(ptr(p), x, y, ptr(o[0:]), ptr(o[1:]), ptr(o[2:]), ptr(o[3:]))
     return d, o[0], o[1], o[2], o[3]
+
+
+def sector_dist(g, x, y):
+    """(d, ex, ey, nx, ny, kind) for the annular sector g = (r0, r1, cos a, sin a, cos b, sin b)."""
+    o = np.zeros(4)
+    k = np.zeros(1, dtype=np.int32)
+    d = lib().orc_sector_dist(ptr(np.ascontiguousarray(g, dtype=np.float64)), x, y, ptr(o[0:1]), ptr(o[1:2]),
+                              ptr(o[2:3]), ptr(o[3:4]), ptr(k, _i))
+    return (d, o[0], o[1], o[2], o[3], int(k[0]))
 
 
 def problem(pde, k2=0.0, lut=None):
@@ -186,8 +201,16 @@ def stencil_ipsi(px, py, c):
     return st, ip.reshape(m, m), cond[0]
 
 
-def simulate(pb, run, x0, y0, rect, knot, path0, npaths, phase2_out=False):
+def simulate(pb, run, x0, y0, rect, knot, path0, npaths, phase2_out=False, geom=0):
+    """Phase-1 paths (phase 2 as well on rectangles); geom 1: rect is an annular sector (6 values)."""
     rect = np.ascontiguousarray(rect, dtype=np.float64)
+    if geom:
+        o = {k: np.zeros(npaths) for k in ("xt", "yt", "Y", "Z", "xi")}
+        o["steps"] = np.zeros(npaths, dtype=np.int64)
+        lib().orc_simulate_geom(C.byref(pb), C.byref(run), x0, y0, geom, ptr(rect), knot, path0, npaths,
+                                ptr(o["xt"]), ptr(o["yt"]), ptr(o["Y"]), ptr(o["Z"]), ptr(o["xi"]),
+                                ptr(o["steps"], _i64))
+        return o
     o = {k: np.zeros(npaths) for k in ("xt", "yt", "Y", "Z", "xi")}
     o["steps"] = np.zeros(npaths, dtype=np.int64)
     p2 = {k: (np.zeros(npaths) if phase2_out else None) for k in ("xt", "yt", "Y", "Z", "xi")}
@@ -200,7 +223,7 @@ def simulate(pb, run, x0, y0, rect, knot, path0, npaths, phase2_out=False):
     return o
 
 
-def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None, w=None):
+def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None, w=None, geom=None):
     msz = np.ascontiguousarray(st["msz"], dtype=np.int32)
     idx = np.ascontiguousarray(st["idx"], dtype=np.int32)
     tot = int(msz.sum())
@@ -216,8 +239,11 @@ def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None, w
     args = [C.byref(pb), n]
     if S is not None:
         args += [S, P if P else n]
-    args += [ptr(paths["xt"]), ptr(paths["yt"]), ptr(paths["Y"]), ptr(paths["Z"]), ptr(paths["xi"]),
-             ptr(np.ascontiguousarray(st["rect"], dtype=np.float64)), ptr(np.ascontiguousarray(glob, dtype=np.float64)),
+    args += [ptr(paths["xt"]), ptr(paths["yt"]), ptr(paths["Y"]), ptr(paths["Z"]), ptr(paths["xi"])]
+    dom = st["domain"] if geom else st["rect"]
+    if geom is not None:
+        args += [int(geom)]
+    args += [ptr(np.ascontiguousarray(dom, dtype=np.float64)), ptr(np.ascontiguousarray(glob, dtype=np.float64)),
              ptr(msz, _i), ptr(idx, _i), ptr(np.ascontiguousarray(st["px"])), ptr(np.ascontiguousarray(st["py"])),
              ptr(cs), ptr(ipsi), quirk]
     eq = np.zeros(2)
@@ -244,12 +270,19 @@ def reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, P=None, quirk=1):
     return _reduce(lib().orc_reduce_batched, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P)
 
 
-def reduce_batched_eq(pb, paths, st, glob, cshape, ipsi_list, S, P, u_ref, quirk=1):
+def reduce_batched_eq(pb, paths, st, glob, cshape, ipsi_list, S, P, u_ref, quirk=1, geom=0):
     """reduce_batched plus the knot equation's mean and batch-means per-path variance with
-    reference interface values u_ref (indexed by knot), summed in the engine's order."""
-    cols = reduce_batched(pb, paths, st, glob, cshape, ipsi_list, S, P, quirk)["cols"]
-    w = np.asarray(u_ref, dtype=np.float64)[cols]
-    return _reduce(lib().orc_reduce_batched_eq, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P, w=w)
+    reference interface values u_ref (indexed by knot), summed in the engine's order.  geom 1:
+    st["domain"] is an annular sector of a disc of radius glob[2]."""
+    w = None
+    if u_ref is not None:
+        cols = _reduce(lib().orc_reduce_batched_geom, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P,
+                       w=np.zeros(int(np.sum(st["msz"]))), geom=geom)["cols"]
+        w = np.asarray(u_ref, dtype=np.float64)[cols]
+    else:
+        w = np.zeros(int(np.sum(st["msz"])))
+    return _reduce(lib().orc_reduce_batched_geom, pb, paths, st, glob, cshape, ipsi_list, quirk, S=S, P=P, w=w,
+                   geom=geom)
 
 
 def bicgstab(rowptr, cols, vals, b, tol=1e-14, maxit=None):

@@ -107,6 +107,10 @@ EXPORTS = {
     "pdds_lut_eval": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, _d, _d, _d]),
     "pdds_mesh_create": (C.c_int32, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, _d,
                                      C.POINTER(C.c_void_p)]),
+    "pdds_mesh_create_disc": (C.c_int32, [C.c_double, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
+                                          C.c_double, C.POINTER(C.c_void_p)]),
+    "pdds_mesh_geometry": (C.c_int32, [C.c_void_p, C.c_int32, _i32, _d]),
+    "pdds_mesh_shape": (C.c_int32, [C.c_void_p, C.c_int32, _d]),
     "pdds_mesh_destroy": (None, [C.c_void_p]),
     "pdds_mesh_n_knots": (C.c_int32, [C.c_void_p]),
     "pdds_mesh_n_stencils": (C.c_int32, [C.c_void_p]),

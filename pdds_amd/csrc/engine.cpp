@@ -156,6 +156,9 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     out.wg = wg;
     out.kstats.assign(knots.size(), KnotStat{});
     if (knots.empty()) return PDDS_OK;
+    const int geom = tabs[0]->geom;
+    for (const DevTables *t : tabs)
+        if (t->geom != geom) return fail(e, PDDS_ERR_INVALID, "one assembly mixes stencil geometries");
     // LDS-tiled form (coefficient LUT with a fifth-field quad plane, persistent launch): knots
     // are grouped into patches of equal tile, tiles that fit in LDS first, in first-seen order
     const bool tiled = persistent && prob->pde == PDDS_PDE_VARCOEF_LUT && r->lut_lds != 0 && pp.lut.nf == 5;
@@ -168,7 +171,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         std::vector<int> rank(knots.size());
         for (size_t i = 0; i < knots.size(); ++i) {
             int c0, r0, tw, th;
-            lut_tile_of(pp.lut, host_sd(knots[i]).rect, c0, r0, tw, th);
+            lut_tile_of(pp.lut, host_sd(knots[i]).bbox, c0, r0, tw, th);
             const int fits = (size_t)tw * th * sizeof(float4) <= tile_cap ? 1 : 0;
             tkey[i] = {c0, r0, tw, th, fits};
             const std::array<int, 4> k4 = fits ? std::array<int, 4>{c0, r0, tw, th} : std::array<int, 4>{-1, -1, -1, -1};
@@ -320,11 +323,11 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             const int g = persistent ? grid : (int)std::min<int64_t>(grid, (tot - base + wg - 1) / wg);
             hipError_t lr;
             if (tiled)
-                lr = r->precision == PDDS_FP64 ? launch_path_tiled_f64(mixed, g, lds_bytes, e->stream, pa)
-                                               : launch_path_tiled_f32(mixed, g, lds_bytes, e->stream, pa);
+                lr = r->precision == PDDS_FP64 ? launch_path_tiled_f64(geom, mixed, g, lds_bytes, e->stream, pa)
+                                               : launch_path_tiled_f32(geom, mixed, g, lds_bytes, e->stream, pa);
             else
-                lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, persistent, mixed, g, wg, e->stream, pa)
-                                               : launch_path_f32(prob->pde, persistent, mixed, g, wg, e->stream, pa);
+                lr = r->precision == PDDS_FP64 ? launch_path_f64(prob->pde, geom, persistent, mixed, g, wg, e->stream, pa)
+                                               : launch_path_f32(prob->pde, geom, persistent, mixed, g, wg, e->stream, pa);
             if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("path launch: ") + hipGetErrorString(lr));
         }
         HIPCHK(hipEventRecord(e->ev[1], e->stream));
@@ -376,8 +379,10 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
     for (const Stencil &s : m.stencils) {
         StencilDesc d;
         std::memset(&d, 0, sizeof(d));
+        for (int i = 0; i < 6; ++i) d.geo[i] = s.geo[i];
         for (int i = 0; i < 4; ++i) {
             d.rect[i] = s.rect[i];
+            d.bbox[i] = s.bbox[i];
             d.inv_c2[i] = 1.0 / (s.cshape[i] * s.cshape[i]);
             d.msz[i] = (int)s.idx[i].size();
         }
@@ -415,6 +420,7 @@ pdds_status mesh_dev_tables(pdds_engine *e, const pdds_mesh *m, int quirk, DevTa
     if (st) return st;
     t.mesh_id = m->id;
     t.quirk = q;
+    t.geom = m->m.geom;
     *out = &t;
     return PDDS_OK;
 }
@@ -628,6 +634,41 @@ pdds_status pdds_mesh_create(double sw_x, double sw_y, double ne_x, double ne_y,
     return m->m.any_singular ? PDDS_ERR_SINGULAR : PDDS_OK;
 }
 
+pdds_status pdds_mesh_create_disc(double radius, int32_t n_rings, int32_t n_sectors, uint32_t seed, int32_t kmin,
+                                  int32_t kmax, double shape, pdds_mesh **out) {
+    if (!out) return PDDS_ERR_INVALID;
+    *out = nullptr;
+    if (!(radius > 0.0) || n_rings < 2 || n_rings > 1024 || n_sectors < 6 || n_sectors > 4096 || kmin < 1 ||
+        kmax < kmin || kmax > 100000 || !(shape > 0.0))
+        return PDDS_ERR_INVALID;
+    pdds_mesh *m = new pdds_mesh(Mesh::disc(radius, n_rings, n_sectors, seed, kmin, kmax, shape));
+    *out = m;
+    return m->m.any_singular ? PDDS_ERR_SINGULAR : PDDS_OK;
+}
+
+pdds_status pdds_mesh_geometry(const pdds_mesh *m, int32_t knot, int32_t *geom, double domain[6]) {
+    if (!m || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
+    const Mesh &M = m->m;
+    if (geom) *geom = M.geom;
+    if (domain) {
+        for (int i = 0; i < 6; ++i) domain[i] = 0.0;
+        const int t = M.stencil_of[knot];
+        if (t >= 0) {
+            const Stencil &s = M.stencils[t];
+            if (M.geom == GeomDisc) for (int i = 0; i < 6; ++i) domain[i] = s.geo[i];
+            else for (int i = 0; i < 4; ++i) domain[i] = s.rect[i];
+        }
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_mesh_shape(const pdds_mesh *m, int32_t knot, double c[4]) {
+    if (!m || !c || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
+    const int t = m->m.stencil_of[knot];
+    for (int i = 0; i < 4; ++i) c[i] = t >= 0 ? m->m.stencils[t].cshape[i] : 0.0;
+    return PDDS_OK;
+}
+
 void pdds_mesh_destroy(pdds_mesh *m) { delete m; }
 int32_t pdds_mesh_n_knots(const pdds_mesh *m) { return m ? m->m.n_knots() : 0; }
 int32_t pdds_mesh_n_stencils(const pdds_mesh *m) { return m ? (int32_t)m->m.stencils.size() : 0; }
@@ -650,7 +691,8 @@ pdds_status pdds_mesh_knots(const pdds_mesh *m, double *x0, double *y0, double *
 pdds_status pdds_mesh_stencil(const pdds_mesh *m, int32_t knot, double rect[4], int32_t msz[4], int32_t kind[4],
                               int32_t cap, int32_t *idx, double *px, double *py, int32_t *n_entries) {
     if (!m || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
-    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    static const Stencil none{};
+    const Stencil &s = m->m.stencil_of[knot] >= 0 ? m->m.stencils[m->m.stencil_of[knot]] : none;
     int tot = s.total();
     if (n_entries) *n_entries = tot;
     for (int i = 0; i < 4; ++i) {
@@ -673,14 +715,16 @@ pdds_status pdds_mesh_stencil(const pdds_mesh *m, int32_t knot, double rect[4], 
 
 pdds_status pdds_mesh_cond(const pdds_mesh *m, int32_t knot, double cond[4]) {
     if (!m || !cond || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
-    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    static const Stencil none{};
+    const Stencil &s = m->m.stencil_of[knot] >= 0 ? m->m.stencils[m->m.stencil_of[knot]] : none;
     for (int i = 0; i < 4; ++i) cond[i] = s.cond[i];
     return PDDS_OK;
 }
 
 pdds_status pdds_mesh_ipsi(const pdds_mesh *m, int32_t knot, int32_t side, int64_t cap, double *out, int32_t *msz) {
     if (!m || knot < 0 || knot >= m->m.n_knots() || side < 0 || side > 3) return PDDS_ERR_INVALID;
-    const Stencil &s = m->m.stencils[m->m.stencil_of[knot]];
+    static const Stencil none{};
+    const Stencil &s = m->m.stencil_of[knot] >= 0 ? m->m.stencils[m->m.stencil_of[knot]] : none;
     const int64_t mm = (int64_t)s.idx[side].size();
     if (msz) *msz = (int32_t)mm;
     if (out) {
@@ -1015,7 +1059,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     StencilDesc d;
     std::memset(&d, 0, sizeof(d));
-    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.inv_c2[i] = 1.0; }
+    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.bbox[i] = glob[i]; d.inv_c2[i] = 1.0; }
     d.gmask = 15;
     std::vector<StencilDesc> sten = {d};
     std::vector<double> entx, enty, ipsi;

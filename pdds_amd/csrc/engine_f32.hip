@@ -5,42 +5,51 @@
 
 namespace pdds {
 
-template <int PDE>
-static hipError_t path_f32(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
+template <int PDE, int GEOM>
+static hipError_t path_f32_g(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     if (!persistent)
-        hipLaunchKernelGGL((plain_kernel<float, PDE>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((plain_kernel<float, PDE, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     else if (mixed)
-        hipLaunchKernelGGL((path_kernel<float, PDE, true>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((path_kernel<float, PDE, true, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     else
-        hipLaunchKernelGGL((path_kernel<float, PDE, false>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((path_kernel<float, PDE, false, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_path_f32(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+template <int PDE>
+static hipError_t path_f32(int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+                          const PathKernelArgs &a) {
+    return geom == GeomDisc ? path_f32_g<PDE, GeomDisc>(persistent, mixed, grid, wg, s, a)
+                            : path_f32_g<PDE, GeomRect>(persistent, mixed, grid, wg, s, a);
+}
+
+hipError_t launch_path_f32(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                            const PathKernelArgs &a) {
     switch (pde) {
-        case PDE_MONEGROS: return path_f32<PDE_MONEGROS>(persistent, mixed, grid, wg, s, a);
-        case PDE_SINSIN: return path_f32<PDE_SINSIN>(persistent, mixed, grid, wg, s, a);
-        case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(persistent, mixed, grid, wg, s, a);
-        case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(persistent, mixed, grid, wg, s, a);
-        case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(persistent, mixed, grid, wg, s, a);
+        case PDE_MONEGROS: return path_f32<PDE_MONEGROS>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_SINSIN: return path_f32<PDE_SINSIN>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(geom, persistent, mixed, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <bool MIXED>
+template <bool MIXED, int GEOM>
 static hipError_t path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float, MIXED>,
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float, MIXED, GEOM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED>), dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED, GEOM>), dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_path_tiled_f32(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
-    return mixed ? path_tiled_f32<true>(grid, lds, s, a) : path_tiled_f32<false>(grid, lds, s, a);
+hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    if (geom == GeomDisc)
+        return mixed ? path_tiled_f32<true, GeomDisc>(grid, lds, s, a) : path_tiled_f32<false, GeomDisc>(grid, lds, s, a);
+    return mixed ? path_tiled_f32<true, GeomRect>(grid, lds, s, a) : path_tiled_f32<false, GeomRect>(grid, lds, s, a);
 }
 
 template <int PDE>

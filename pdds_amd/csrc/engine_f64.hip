@@ -8,44 +8,53 @@
 
 namespace pdds {
 
-template <int PDE>
-static hipError_t path_f64(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
+template <int PDE, int GEOM>
+static hipError_t path_f64_g(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     if (!persistent)
-        hipLaunchKernelGGL((plain_kernel<double, PDE>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((plain_kernel<double, PDE, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     else if (mixed)
-        hipLaunchKernelGGL((path_kernel<double, PDE, true>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((path_kernel<double, PDE, true, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     else
-        hipLaunchKernelGGL((path_kernel<double, PDE, false>), dim3(grid), dim3(wg), 0, s, a);
+        hipLaunchKernelGGL((path_kernel<double, PDE, false, GEOM>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_path_f64(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+template <int PDE>
+static hipError_t path_f64(int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+                          const PathKernelArgs &a) {
+    return geom == GeomDisc ? path_f64_g<PDE, GeomDisc>(persistent, mixed, grid, wg, s, a)
+                            : path_f64_g<PDE, GeomRect>(persistent, mixed, grid, wg, s, a);
+}
+
+hipError_t launch_path_f64(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                            const PathKernelArgs &a) {
     switch (pde) {
-        case PDE_MONEGROS: return path_f64<PDE_MONEGROS>(persistent, mixed, grid, wg, s, a);
-        case PDE_SINSIN: return path_f64<PDE_SINSIN>(persistent, mixed, grid, wg, s, a);
-        case PDE_VARCOEF: return path_f64<PDE_VARCOEF>(persistent, mixed, grid, wg, s, a);
-        case PDE_VARCOEF_LUT: return path_f64<PDE_VARCOEF_LUT>(persistent, mixed, grid, wg, s, a);
-        case PDE_HELMHOLTZ: return path_f64<PDE_HELMHOLTZ>(persistent, mixed, grid, wg, s, a);
+        case PDE_MONEGROS: return path_f64<PDE_MONEGROS>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_SINSIN: return path_f64<PDE_SINSIN>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF: return path_f64<PDE_VARCOEF>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_VARCOEF_LUT: return path_f64<PDE_VARCOEF_LUT>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_HELMHOLTZ: return path_f64<PDE_HELMHOLTZ>(geom, persistent, mixed, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }
 
 int path_tiled_wg(int precision) { return precision == PDDS_FP64 ? tiled_wg<double>() : tiled_wg<float>(); }
 
-template <bool MIXED>
+template <bool MIXED, int GEOM>
 static hipError_t path_tiled_f64(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<double, MIXED>,
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<double, MIXED, GEOM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED>), dim3(grid), dim3(tiled_wg<double>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED, GEOM>), dim3(grid), dim3(tiled_wg<double>()), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_path_tiled_f64(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
-    return mixed ? path_tiled_f64<true>(grid, lds, s, a) : path_tiled_f64<false>(grid, lds, s, a);
+hipError_t launch_path_tiled_f64(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    if (geom == GeomDisc)
+        return mixed ? path_tiled_f64<true, GeomDisc>(grid, lds, s, a) : path_tiled_f64<false, GeomDisc>(grid, lds, s, a);
+    return mixed ? path_tiled_f64<true, GeomRect>(grid, lds, s, a) : path_tiled_f64<false, GeomRect>(grid, lds, s, a);
 }
 
 template <int PDE>
@@ -73,7 +82,9 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 // (the order of the per-exit loop it replaces), two exits per iteration so that two exp chains
 // are in flight; Phi is kept [entry][sub-batch] so that the Psi^-T pass reads a row of kNBG
 // values with two 16-byte LDS reads per pair of sub-batches.
-__global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
+// EQ: the launch carries reference values (ReduceArgs::wref) and also sums the knot equations
+template <bool EQ>
+__global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(4))) reduce_kernel(ReduceArgs a) {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
     __shared__ double s_wsum[(kReduceWG / 64) * kNBG];
@@ -253,13 +264,13 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
             }
             accS[c] = sum;
             accQ[c] = q2;
-            if (a.wref) {
+            if constexpr (EQ) {
                 const double wc = a.wref[kd.out_off + c];
 #pragma unroll
                 for (int b = 0; b < kNBG; ++b) tp[b] = tp[b] + G[b] * wc;
             }
         }
-        if (a.wref) {
+        if constexpr (EQ) {
             // sum over the columns: a butterfly over each wave, then the waves in order
 #pragma unroll
             for (int b = 0; b < kNBG; ++b) {
@@ -271,7 +282,7 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
         }
         if (tid == 0) {
             for (int b = 0; b < nb; ++b) { Bs = Bs + s_bsb[b]; BBs = BBs + s_bbsb[b]; }
-            if (a.wref)
+            if constexpr (EQ)
                 for (int b = 0; b < nb; ++b) {
                     double T = 0.0;
                     for (int w = 0; w < nwaves; ++w) T = T + s_wsum[w * kNBG + b];
@@ -303,7 +314,10 @@ __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a) {
 }
 
 hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
-    hipLaunchKernelGGL(reduce_kernel, dim3(grid), dim3(wg), shmem, s, a);
+    if (a.wref)
+        hipLaunchKernelGGL(reduce_kernel<true>, dim3(grid), dim3(wg), shmem, s, a);
+    else
+        hipLaunchKernelGGL(reduce_kernel<false>, dim3(grid), dim3(wg), shmem, s, a);
     return hipGetLastError();
 }
 

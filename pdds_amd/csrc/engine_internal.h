@@ -24,6 +24,7 @@ struct DevBuf {
 struct DevTables {
     uint64_t mesh_id = 0;
     int quirk = -1;
+    int geom = 0;                         // GeomRect / GeomDisc stopping domains
     int n_sten = 0;
     StencilDesc *sten = nullptr;
     double *entx = nullptr, *enty = nullptr, *ipsi = nullptr;
@@ -36,7 +37,7 @@ struct DevTables {
         for (void *p : ps)
             if (p) (void)hipFree(p);
         sten = nullptr; entx = enty = ipsi = nullptr; ents = nullptr; e1 = e2 = nullptr;
-        mesh_id = 0; quirk = -1; n_sten = 0; host_sten.clear(); bytes = 0;
+        mesh_id = 0; quirk = -1; geom = 0; n_sten = 0; host_sten.clear(); bytes = 0;
     }
 };
 
@@ -70,6 +71,7 @@ struct pdds_mesh {
     uint64_t id;
     pdds_mesh(double a, double b, double c, double d, int n, int k, const double f[4])
         : m(a, b, c, d, n, k, f), id(next_mesh_id()) {}
+    explicit pdds_mesh(Mesh &&mm) : m(std::move(mm)), id(next_mesh_id()) {}
 };
 
 enum BufSlot {

@@ -51,6 +51,19 @@ PDDS_HD int classify_coord(const double rect[4], const double glob[4], double x,
     return -1;
 }
 
+// Exit classification of FP64 paths on an annular-sector stencil g (sector_dist) of a disc of
+// radius R centred at the origin: the global test (on the disc boundary) first, then inner arc,
+// outer arc, ray a, ray b -- the order of the rectangle's S, N, W, E test.
+PDDS_HD int classify_sector(const double g[6], double R, double x, double y) {
+    const double r = __builtin_sqrt(x * x + y * y);
+    if (r >= R - 1e-8) return 4;
+    if (g[0] > 0.0 && __builtin_fabs(r - g[0]) < 1e-8) return 1;
+    if (__builtin_fabs(r - g[1]) < 1e-8) return 0;
+    if (__builtin_fabs(g[2] * y - g[3] * x) < 1e-8) return 3;
+    if (__builtin_fabs(x * g[5] - y * g[4]) < 1e-8) return 2;
+    return -1;
+}
+
 // Exit classification of FP32 paths from the integer projection kind: a side of
 // the stencil rectangle that lies on the global boundary (mask bit s) gives a
 // global exit; a corner is global if either adjacent side is, else it takes the
@@ -165,12 +178,16 @@ __device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int
 // (Z + xi, plus Y g(X) for a global-boundary exit, Reduce_Analytic
 // Solvers/EMFKACSolver.hpp:559-566), the exit side and the step count (kCappedBit: stopped by
 // the step cap while still inside).
-template <class R, int PDE>
+template <class R, int PDE, int GEOM>
 __device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, const PathState<R> &s,
-                                           const R rect[4], int gmask, R ex, R ey, int kind, bool capped_inside) {
+                                           const R *dom, int gmask, R ex, R ey, int kind, bool capped_inside) {
     int side;
-    if constexpr (sizeof(R) == 8) side = classify_coord(rect, a.glob, (double)ex, (double)ey);
-    else side = classify_kind(kind, gmask);
+    if constexpr (sizeof(R) == 8) {
+        if constexpr (GEOM == GeomDisc) side = classify_sector(dom, a.glob[2], (double)ex, (double)ey);
+        else side = classify_coord(dom, a.glob, (double)ex, (double)ey);
+    } else {
+        side = classify_kind(kind, gmask);
+    }
     double sc;
     if (side == 4) {
         R gv = pde_g<PDE, R>(ex, ey, a.pp);
@@ -186,17 +203,29 @@ __device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, 
     a.rec.steps[g] = (int32_t)s.st | (capped_inside ? kCappedBit : 0);
 }
 
+// Distance to a stencil's stopping domain: its rectangle, or its annular sector on a disc mesh.
+template <class R, int GEOM>
+__device__ __forceinline__ R domain_dist(const R *dom, R x, R y, R &ex, R &ey, R &nx, R &ny, int &kind) {
+    if constexpr (GEOM == GeomDisc) return sector_dist<R>(dom, x, y, ex, ey, nx, ny, kind);
+    else return rect_dist<R>(dom, x, y, ex, ey, nx, ny, kind);
+}
+template <int GEOM> __device__ __forceinline__ const double *domain_of(const StencilDesc &sd) {
+    if constexpr (GEOM == GeomDisc) return sd.geo;
+    else return sd.rect;
+}
+template <int GEOM> constexpr int domain_len() { return GEOM == GeomDisc ? 6 : 4; }
+
 // ----------------------------------------------------------------------------
 // Per-lane walker: the state of the path a lane carries and one step of it
 // ----------------------------------------------------------------------------
 // MIXED: the knots of the launch differ in Philox stream or Helmholtz k2 (serving rounds), so
 // each lane carries its knot's; otherwise both are the launch's and stay in scalar registers.
-template <class R, int PDE, bool MIXED>
+template <class R, int PDE, bool MIXED, int GEOM>
 struct Walker {
     PathState<R> s;
     Coef<R> co;            // coefficients at the current X
     R pn1, pn2;            // normals of step s.st (the next step)
-    R rect[4];             // stencil rectangle in path precision (FP64 classifies on it)
+    R rect[domain_len<GEOM>()];   // stopping domain in path precision (FP64 classifies on it)
     uint32_t kid;
     int gmask;
     uint32_t g;            // batch path index (record slot; a batch holds fewer than 2^32 paths)
@@ -213,7 +242,8 @@ struct Walker {
     __device__ __forceinline__ void start(const PathKernelArgs &a, int64_t gg, int k) {
         const KnotDesc &kd = a.knots[k];
         const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
-        for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
+        const double *dom = domain_of<GEOM>(sd);
+        for (int q = 0; q < domain_len<GEOM>(); ++q) rect[q] = (R)dom[q];
         gmask = sd.gmask;
         kid = kd.knot_id;
         if constexpr (MIXED) {
@@ -260,7 +290,7 @@ struct Walker {
         }
         R ex, ey, nx, ny;
         int kind;
-        const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
         pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, kk2, co);
         if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
@@ -269,7 +299,7 @@ struct Walker {
         const bool go = d < beta * nrm * sqrth;
         const bool capped = (int64_t)s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
-            write_exit<R, PDE>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
+            write_exit<R, PDE, GEOM>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
             active = false;
         }
         fresh = false;
@@ -325,11 +355,11 @@ struct WavePool {
 // ----------------------------------------------------------------------------
 // Persistent path kernel: one path per lane, lanes refilled from XCD-local queues
 // ----------------------------------------------------------------------------
-template <class R, int PDE, bool MIXED>
+template <class R, int PDE, bool MIXED, int GEOM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
     const int lane = __lane_id();
     const LutTile none{nullptr, 0, 0, 0, 0};
-    Walker<R, PDE, MIXED> w;
+    Walker<R, PDE, MIXED, GEOM> w;
     w.init();
     bool done = false;
     // The wave starts on its XCD's queue, so the paths (and LUT texels) of one XCD's L2
@@ -365,7 +395,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
 // its chunked relaunches, Solvers/SolveLoop.cuh:20-887): one work-item per path of the batch,
 // each stepping its own path to the exit; a wave lasts as long as its slowest path.
 // ----------------------------------------------------------------------------
-template <class R, int PDE>
+template <class R, int PDE, int GEOM>
 __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
     const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gg >= a.total) return;
@@ -373,8 +403,9 @@ __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
     const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
     const LutTile none{nullptr, 0, 0, 0, 0};
     const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta, k2 = (R)kd.k2;
-    R rect[4];
-    for (int q = 0; q < 4; ++q) rect[q] = (R)sd.rect[q];
+    R rect[domain_len<GEOM>()];
+    const double *dom = domain_of<GEOM>(sd);
+    for (int q = 0; q < domain_len<GEOM>(); ++q) rect[q] = (R)dom[q];
     PathState<R> s;
     s.path = (uint32_t)(gg - kd.path_off);
     s.x = (R)kd.x0; s.y = (R)kd.y0;
@@ -390,7 +421,7 @@ __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
         draw_normals<R>(s, a.seed, kd.stream, kd.knot_id, n1, n2, (s.st & 1) == 0);
         em_step<R>(s, co, h, sqrth, n1, n2);
         s.st += 1;
-        const R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
         pde_coef_t<PDE, R, false>(s.x, s.y, a.pp, none, k2, co);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
@@ -398,7 +429,7 @@ __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
         go = d < beta * nrm * sqrth;
         capped = (int64_t)s.st >= a.max_steps;
     } while (go && !capped);
-    write_exit<R, PDE>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind, capped && go);
+    write_exit<R, PDE, GEOM>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind, capped && go);
 }
 
 // ----------------------------------------------------------------------------
@@ -420,13 +451,13 @@ __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) 
     return lo;
 }
 
-template <class R, bool MIXED>
+template <class R, bool MIXED, int GEOM>
 __global__ void __launch_bounds__(tiled_wg<R>()) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE_VARCOEF_LUT>())))
 path_kernel_tiled(PathKernelArgs a) {
     extern __shared__ float4 tile[];
     __shared__ int s_patch;
     const int lane = __lane_id();
-    Walker<R, PDE_VARCOEF_LUT, MIXED> w;
+    Walker<R, PDE_VARCOEF_LUT, MIXED, GEOM> w;
     w.init();
     // first patch: the one holding this work-group's share of the batch's path range
     int p = find_patch(a.patches, a.n_patches, (int64_t)blockIdx.x * a.total / (int64_t)gridDim.x);
@@ -480,7 +511,7 @@ path_kernel_tiled(PathKernelArgs a) {
 // ----------------------------------------------------------------------------
 // Reduction kernel: one work-group per work item
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a);
+template <bool EQ> __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a);
 
 // One block per knot: sum the knot's items in item order.
 __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);

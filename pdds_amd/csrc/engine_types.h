@@ -38,7 +38,9 @@ struct KnotDesc {
 // One distinct stencil: stopping rectangle, four side lists (N,S,E,W), Psi^-1 per side,
 // and for each CSR column of the row up to two contributing side entries.
 struct StencilDesc {
-    double rect[4];
+    double rect[4];         // rectangular stopping domain x0, y0, x1, y1
+    double geo[6];          // annular-sector stopping domain r0, r1, cos/sin ray a, cos/sin ray b
+    double bbox[4];         // bounding box of the stopping domain (host: LUT tiles)
     double inv_c2[4];       // 1 / c^2 of the Gaussian shape c per side
     int64_t ipsi_off[4];    // row-major m x m blocks in the ipsi array
     int32_t msz[4];
@@ -164,12 +166,12 @@ struct PathsArgs {
 };
 
 // Launchers (defined in engine_f64.hip / engine_f32.hip)
-hipError_t launch_path_tiled_f64(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
-hipError_t launch_path_tiled_f32(bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_tiled_f64(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
+hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
 int path_tiled_wg(int precision);
-hipError_t launch_path_f64(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+hipError_t launch_path_f64(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                            const PathKernelArgs &a);
-hipError_t launch_path_f32(int pde, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
+hipError_t launch_path_f32(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                            const PathKernelArgs &a);
 hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a);
 hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const PathsArgs &a);

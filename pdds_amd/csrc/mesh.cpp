@@ -132,8 +132,7 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
         int kd;
         on_boundary[q] = std::fabs(rect_dist<double>(glob, x0[q], y0[q], ex, ey, nx, ny, kd)) <= 1e-8;
     }
-    // stencils of every knot (independent, spread over host threads), then deduplicated in knot
-    // order: a knot takes the first earlier stencil equal to its own
+    // stencils of every knot (independent, spread over host threads)
     const unsigned hw = std::thread::hardware_concurrency();
     const size_t nth_all = std::min<size_t>((size_t)(hw ? hw : 1), 16);
     std::vector<Stencil> per_knot(nknots_);
@@ -147,17 +146,36 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
         work();
         for (std::thread &th : pool) th.join();
     }
+    for (Stencil &s : per_knot) {
+        for (int d = 0; d < 4; ++d) s.cshape[d] = fac[s.kind[d] < 0 ? 0 : (s.kind[d] > 3 ? 3 : s.kind[d])];
+        for (int i = 0; i < 4; ++i) s.bbox[i] = s.rect[i];
+        if (std::fabs(s.rect[0] - glob[0]) < 1e-8) s.gmask |= 1 << West;
+        if (std::fabs(s.rect[1] - glob[1]) < 1e-8) s.gmask |= 1 << South;
+        if (std::fabs(s.rect[2] - glob[2]) < 1e-8) s.gmask |= 1 << East;
+        if (std::fabs(s.rect[3] - glob[3]) < 1e-8) s.gmask |= 1 << North;
+    }
+    std::vector<char> has(nknots_, 1);
+    finalize(per_knot, has);
+}
+
+void Mesh::finalize(std::vector<Stencil> &per_knot, const std::vector<char> &has_stencil) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth_all = std::min<size_t>((size_t)(hw ? hw : 1), 16);
+    // deduplicated in knot order: a knot takes the first earlier stencil equal to its own
     stencil_of.assign(nknots_, -1);
     std::map<std::vector<int64_t>, int> seen;
     for (int q = 0; q < nknots_; ++q) {
+        if (!has_stencil[q]) continue;
         const Stencil &s = per_knot[q];
         std::vector<int64_t> key;
         for (int i = 0; i < 4; ++i) {
             key.push_back((int64_t)f64_bits(s.rect[i]));
+            key.push_back((int64_t)f64_bits(s.cshape[i]));
             key.push_back(s.kind[i]);
             key.push_back((int64_t)s.idx[i].size());
             key.insert(key.end(), s.idx[i].begin(), s.idx[i].end());
         }
+        for (int i = 0; i < 6; ++i) key.push_back((int64_t)f64_bits(s.geo[i]));
         auto it = seen.find(key);
         if (it == seen.end()) {
             it = seen.emplace(std::move(key), (int)stencils.size()).first;
@@ -170,18 +188,19 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
     std::vector<std::pair<int, int>> jobs;
     for (int t = 0; t < (int)stencils.size(); ++t)
         for (int d = 0; d < 4; ++d)
-            if (!stencils[t].idx[d].empty()) {
-                Stencil &s = stencils[t];
-                s.cshape[d] = fac[s.kind[d] < 0 ? 0 : (s.kind[d] > 3 ? 3 : s.kind[d])];
-                jobs.emplace_back(t, d);
-            }
+            if (!stencils[t].idx[d].empty()) jobs.emplace_back(t, d);
     std::vector<char> ok(jobs.size(), 1);
     std::atomic<size_t> next{0};
     auto work = [&]() {
         for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
             Stencil &s = stencils[jobs[j].first];
             const int d = jobs[j].second;
-            ok[j] = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]) ? 1 : 0;
+            bool good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]);
+            for (int t = 0; s.cond_max > 0.0 && t < 40 && (!good || s.cond[d] > s.cond_max); ++t) {
+                s.cshape[d] *= 0.8;
+                good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]);
+            }
+            ok[j] = good && !(s.cond_max > 0.0 && s.cond[d] > s.cond_max) ? 1 : 0;
         }
     };
     const size_t nth = std::min<size_t>(nth_all, jobs.size() / 4 + 1);
@@ -191,7 +210,7 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
     for (std::thread &th : pool) th.join();
     for (size_t j = 0; j < jobs.size(); ++j)
         if (!ok[j]) { stencils[jobs[j].first].singular = true; any_singular = true; }
-    // per stencil: CSR row, column -> side entry maps, global-side mask
+    // per stencil: CSR row and column -> side entry maps
     for (Stencil &s : stencils) {
         std::vector<int> ent;
         for (int d = 0; d < 4; ++d) ent.insert(ent.end(), s.idx[d].begin(), s.idx[d].end());
@@ -206,19 +225,16 @@ Mesh::Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_
             if (s.e1_sum[c] < 0) s.e1_sum[c] = e;
             else s.e2_sum[c] = e;
         }
-        if (std::fabs(s.rect[0] - glob[0]) < 1e-8) s.gmask |= 1 << West;
-        if (std::fabs(s.rect[1] - glob[1]) < 1e-8) s.gmask |= 1 << South;
-        if (std::fabs(s.rect[2] - glob[2]) < 1e-8) s.gmask |= 1 << East;
-        if (std::fabs(s.rect[3] - glob[3]) < 1e-8) s.gmask |= 1 << North;
     }
     row_ptr.assign(nknots_ + 1, 0);
     for (int q = 0; q < nknots_; ++q) {
-        int nc = on_boundary[q] ? 0 : (int)stencils[stencil_of[q]].cols.size();
+        int nc = (on_boundary[q] || stencil_of[q] < 0) ? 0 : (int)stencils[stencil_of[q]].cols.size();
         row_ptr[q + 1] = row_ptr[q] + nc;
     }
     cols.resize(row_ptr[nknots_]);
     for (int q = 0; q < nknots_; ++q)
-        if (!on_boundary[q]) std::copy(stencils[stencil_of[q]].cols.begin(), stencils[stencil_of[q]].cols.end(), cols.begin() + row_ptr[q]);
+        if (row_ptr[q + 1] > row_ptr[q])
+            std::copy(stencils[stencil_of[q]].cols.begin(), stencils[stencil_of[q]].cols.end(), cols.begin() + row_ptr[q]);
 }
 
 // Labels_Stencil (PDDAlgorithms/PDDSparseGM.cpp:1330-1433)

@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include "pdds_math.h"
+
 namespace pdds {
 
 enum Dir { North = 0, South = 1, East = 2, West = 3 };
@@ -35,7 +37,10 @@ struct Interface {
 };
 
 struct Stencil {
-    double rect[4];
+    double rect[4];                        // rectangle x0, y0, x1, y1 (rectangular meshes)
+    // annular sector r0, r1, cos/sin of the start ray a, cos/sin of the end ray b (disc meshes)
+    double geo[6] = {0, 0, 0, 0, 0, 0};
+    double bbox[4] = {0, 0, 0, 0};         // bounding box of the stopping domain (LUT tiles)
     int kind[4];
     std::vector<int> idx[4];
     std::vector<double> px[4], py[4];
@@ -48,6 +53,8 @@ struct Stencil {
     std::vector<int> e1_sum, e2_sum;       // same, summing shared corners
     int gmask = 0;
     bool singular = false;
+    // > 0: shrink a side's RBF shape by 0.8 until its Psi is invertible with cond below this
+    double cond_max = 0.0;
     int total() const { return (int)(idx[0].size() + idx[1].size() + idx[2].size() + idx[3].size()); }
     bool same_as(const Stencil &o) const;
 };
@@ -55,6 +62,11 @@ struct Stencil {
 class Mesh {
   public:
     Mesh(double sw_x, double sw_y, double ne_x, double ne_y, int n_sub, int n_node, const double fac[4]);
+    // Disc of the given radius cut into n_rings rings x n_sectors sectors; each interface (arc or
+    // radial segment) carries a number of knots drawn uniformly from [kmin, kmax] with `seed`.
+    static Mesh disc(double radius, int n_rings, int n_sectors, uint32_t seed, int kmin, int kmax, double shape);
+    int geom = GeomRect;
+    double radius = 0.0;                    // disc meshes
     int n_knots() const { return nknots_; }
     const std::vector<Interface> &interfaces() const { return ifs_; }
     double glob[4];
@@ -68,10 +80,13 @@ class Mesh {
     double fac[4];
 
   private:
+    Mesh() = default;
+    // dedup per-knot stencils (cshape and gmask set), Psi^-1 per side, CSR rows, column maps
+    void finalize(std::vector<Stencil> &per_knot, const std::vector<char> &has_stencil);
     int find(int a, int b) const;
     std::map<Dir, std::vector<std::array<int, 2>>> labels_stencil(int k) const;
     Stencil compute_stencil(int k) const;
-    int n_, k_;
+    int n_ = 0, k_ = 0;
     int nknots_ = 0;
     std::vector<Interface> ifs_;
     std::map<std::array<int, 2>, int> imap_;

@@ -20,6 +20,9 @@
 
 namespace pdds {
 
+// Stopping-domain geometry of a mesh's stencils: rectangles, or annular sectors of a disc.
+enum Geom { GeomRect = 0, GeomDisc = 1 };
+
 // ------------------------------------------------------------------ FP64 --
 PDDS_HD uint64_t f64_bits(double x) { return (uint64_t)__builtin_bit_cast(uint64_t, x); }
 PDDS_HD double f64_from(uint64_t u) { return __builtin_bit_cast(double, u); }
@@ -353,6 +356,45 @@ PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kin
     ey = sys * hy + cy; nx = 0; ny = sys;
     kind = top ? 0 : 1;
     return dy - hy;
+}
+
+// Signed distance to the annular sector g = {r0, r1, cos a, sin a, cos b, sin b}: radii
+// r0 <= r <= r1 between the rays a and b (b counterclockwise from a, less than pi apart;
+// r0 = 0 for a sector at the centre, which then has no inner side).  Inside, the distance to
+// the nearest side is the largest of the four half-plane / annulus distances (exact for such
+// a sector); outside, that largest violation stands for the distance (positive).  The side it
+// names gives the projection (clamped to the side) and the outward normal; `kind` reports it
+// as 0 outer arc, 1 inner arc, 2 ray b, 3 ray a.
+template <class R>
+PDDS_HD R sector_dist(const R g[6], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kind) {
+    const R r0 = g[0], r1 = g[1], ca = g[2], sa = g[3], cb = g[4], sb = g[5];
+    const R r = RM<R>::sqrt(x * x + y * y);
+    const R sA = ca * y - sa * x;          // > 0: counterclockwise of ray a (inside)
+    const R sB = x * sb - y * cb;          // > 0: clockwise of ray b (inside)
+    R d = r - r1;
+    int k = 0;
+    const R din = r0 > (R)0 ? r0 - r : -(r + r1);
+    if (din > d) { d = din; k = 1; }
+    if (-sB > d) { d = -sB; k = 2; }
+    if (-sA > d) { d = -sA; k = 3; }
+    if (k <= 1) {
+        // radial direction of X, or the nearer ray's when X lies outside the angle
+        R ux = x / r, uy = y / r;
+        if (sA < (R)0) { ux = ca; uy = sa; }
+        else if (sB < (R)0) { ux = cb; uy = sb; }
+        const R rr = k == 0 ? r1 : r0;
+        ex = rr * ux; ey = rr * uy;
+        nx = k == 0 ? ux : -ux; ny = k == 0 ? uy : -uy;
+    } else {
+        const R dx = k == 3 ? ca : cb, dy = k == 3 ? sa : sb;
+        R t = x * dx + y * dy;
+        t = t < r0 ? r0 : (t > r1 ? r1 : t);
+        ex = t * dx; ey = t * dy;
+        nx = k == 3 ? sa : -sb;
+        ny = k == 3 ? -ca : cb;
+    }
+    kind = k;
+    return d;
 }
 
 // --------------------------------------------------------------- LUT -----

@@ -175,7 +175,7 @@ bool run_round(pdds_server *s) {
     if (!s->dom_ready) {
         StencilDesc d;
         std::memset(&d, 0, sizeof(d));
-        for (int i = 0; i < 4; ++i) { d.rect[i] = s->cfg.domain[i]; d.inv_c2[i] = 1.0; }
+        for (int i = 0; i < 4; ++i) { d.rect[i] = s->cfg.domain[i]; d.bbox[i] = s->cfg.domain[i]; d.inv_c2[i] = 1.0; }
         d.gmask = 15;
         std::vector<StencilDesc> sd = {d};
         std::vector<double> none;

@@ -235,10 +235,13 @@ class Lut:
 class Mesh:
     """PDDSparse interfaces, knots and stencils (PDDSparseGM geometry)."""
 
-    def __init__(self, sw, ne, n_sub, n_node, fac):
+    def __init__(self, sw, ne, n_sub, n_node, fac, _disc=None):
         h = C.c_void_p()
-        fac = f64(fac)
-        st = A.lib().pdds_mesh_create(sw[0], sw[1], ne[0], ne[1], n_sub, n_node, ptr(fac), C.byref(h))
+        if _disc is None:
+            fac = f64(fac)
+            st = A.lib().pdds_mesh_create(sw[0], sw[1], ne[0], ne[1], n_sub, n_node, ptr(fac), C.byref(h))
+        else:
+            st = A.lib().pdds_mesh_create_disc(*_disc, C.byref(h))
         if st not in (A.OK, A.ERR_SINGULAR):
             raise PddsError(st, "mesh create")
         self.h = h
@@ -247,6 +250,9 @@ class Mesh:
         self.n_knots = A.lib().pdds_mesh_n_knots(h)
         self.n_stencils = A.lib().pdds_mesh_n_stencils(h)
         self.n_node = n_node
+        g = C.c_int32()
+        A.lib().pdds_mesh_geometry(h, 0, C.byref(g), None)
+        self.geom = g.value
         nnz = C.c_int64()
         A.lib().pdds_mesh_csr(h, None, None, C.byref(nnz))
         self.nnz = nnz.value
@@ -261,6 +267,13 @@ class Mesh:
         A.lib().pdds_mesh_knots(h, ptr(self.x0), ptr(self.y0), ptr(self.px), ptr(self.py),
                                 ptr(self.on_boundary, A._i32), ptr(self.stencil_id, A._i32))
         self.on_boundary = self.on_boundary.astype(bool)
+
+    @classmethod
+    def disc(cls, radius=1.0, n_rings=6, n_sectors=24, seed=11, kmin=8, kmax=64, shape=1.0):
+        """Disc of `radius` cut into n_rings rings x n_sectors sectors, kmin..kmax knots per
+        interface drawn with `seed` (see pdds_mesh_create_disc)."""
+        return cls((-radius, -radius), (radius, radius), 0, 0, None,
+                   _disc=(radius, n_rings, n_sectors, seed, kmin, kmax, shape))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -296,8 +309,12 @@ class Mesh:
             mm = C.c_int32()
             A.lib().pdds_mesh_ipsi(self.h, k, s, a.size, ptr(a), C.byref(mm))
             ipsi.append(a[:m * m].reshape(m, m))
+        dom = np.zeros(6)
+        A.lib().pdds_mesh_geometry(self.h, k, None, ptr(dom))
+        cshape = np.zeros(4)
+        A.lib().pdds_mesh_shape(self.h, k, ptr(cshape))
         return {"rect": rect, "msz": msz, "kind": kind, "idx": idx[:t], "px": px[:t], "py": py[:t], "cond": cond,
-                "ipsi": ipsi}
+                "ipsi": ipsi, "domain": dom, "geom": self.geom, "cshape": cshape}
 
 
 class Server:

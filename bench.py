@@ -186,6 +186,39 @@ def serving_bench(eng, duration_s=1.5, h=1e-3, seed=23, plain=True):
     return out
 
 
+def batching_bench(eng):
+    """The batching configuration: disc mesh, autotuned launch shape, variance-targeted per-knot
+    path counts, every round one persistent launch of uneven counts (FP64 paths)."""
+    from pdds_amd.engine import default_run
+    from pdds_amd import workloads as W
+    t0 = time.perf_counter()
+    mesh, prob, h, seed, cfg = W.build_disc(eng)
+    t_mesh = time.perf_counter() - t0
+    base = default_run(h=h, seed=seed)
+    best, tune = eng.autotune(mesh, prob, base)
+    t1 = time.perf_counter()
+    out = eng.solve_adaptive(mesh, prob, best, **cfg)
+    wall = time.perf_counter() - t1
+    rep = out["report"]
+    mc = ~mesh.on_boundary
+    N = out["paths_per_knot"][mc]
+    err = out["u"] - prob.u(mesh.px, mesh.py)
+    return {
+        "workload": "disc r=1, 6 rings x 24 sectors, 8..64 knots/interface (seed 11): %d knots (%d MC), "
+                    "cfg2 PDE from 512^2 LUTs, h=%g, FP64 paths, SE target %g per knot equation, N in [%d, %d]"
+                    % (mesh.n_knots, int(mc.sum()), h, cfg["se_target"], cfg["n_min"], cfg["n_max"]),
+        "path_steps_per_s": rep["path_steps"] / wall, "seconds": wall, "mesh_build_s": t_mesh,
+        "rounds": rep["rounds"], "paths": rep["paths"], "path_steps": rep["path_steps"],
+        "paths_per_knot": {"min": int(N.min()), "median": int(np.median(N)), "p99": int(np.percentile(N, 99)),
+                           "max": int(N.max())},
+        "max_se": rep["max_se"], "knots_capped": rep["knots_capped"], "kernel_ms": rep["kernel_ms"],
+        "rms_interface_error": float(np.sqrt(np.mean(err ** 2))),
+        "autotune": {"best": {k: getattr(best, k) for k in ("lut_lds", "wg_size", "grid_wgs", "paths_per_lane")},
+                     "candidates": tune["candidates"], "best_rate": tune["best_rate"], "base_rate": tune["base_rate"],
+                     "ms": tune["total_ms"]},
+    }
+
+
 def rms_error(eng, mesh, prob, out):
     u, it, rr = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, out["G"], out["B"], tol=1e-12)
     ex = prob.u(mesh.px, mesh.py)
@@ -203,6 +236,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-only", action="store_true", help="print only the serving scenario")
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--batching-only", action="store_true", help="print only the batching (disc, adaptive) scenario")
     ap.add_argument("--serving-seconds", type=float, default=1.5)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_cfg2.json"))
     ap.add_argument("--copy-calibration", action="store_true",
@@ -224,6 +259,10 @@ def main():
     info = eng.info()
     if args.serving_only:
         print(json.dumps({"serving": serving_bench(eng, args.serving_seconds)}))
+        eng.close()
+        return
+    if args.batching_only:
+        print(json.dumps({"batching": batching_bench(eng)}))
         eng.close()
         return
     mesh, prob, n_paths, h, seed = W.build(args.config, eng)
@@ -306,6 +345,8 @@ def main():
         result["forms"] = extras
     if not args.no_serving and not args.no_extras:
         result["serving"] = serving_bench(eng, args.serving_seconds)
+    if not args.no_extras and not args.no_batching:
+        result["batching"] = batching_bench(eng)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(mesh, {3: "varcoef_lut", 0: "monegros", 1: "sinsin"}[prob.pde],
                                               n_paths, h, seed)

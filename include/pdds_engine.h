@@ -77,7 +77,8 @@ typedef struct {
     uint32_t seed;          /* Philox key word 0 */
     uint32_t stream;        /* Philox key word 1: request id, 0 for plain assembly */
     int32_t precision;      /* PDDS_FP64 or PDDS_FP32 */
-    int32_t quirk_last_side;/* 1: corner knots keep the last side's value (Meshes/stencil.cpp:376-395) */
+    int32_t quirk_last_side;/* 1: corner knots keep the last side's value (Meshes/stencil.cpp:376-395);
+                               rectangular meshes only, disc meshes always sum */
     int64_t max_steps;      /* per-path step cap (0: 2^26) */
     int32_t sub_batch;      /* paths per variance batch S (0: 32) */
     int32_t item_paths;     /* paths per work item P, multiple of S (0: chosen from the path count) */
@@ -111,8 +112,9 @@ pdds_status pdds_mesh_create(double sw_x, double sw_y, double ne_x, double ne_y,
    knots (count from a hash of seed and the interface), with a knot at every crossing of an inner
    circle and a ray and a boundary knot where each ray meets the rim.  A knot's stencil is the
    annular sector of the subdomains around its interface, with sides outer arc (0), inner arc (1),
-   end ray (2) and start ray (3); each side's Gaussian shape starts at shape x 3.5 x its mean knot
-   spacing and shrinks until the side's Psi has cond below 1e11. */
+   end ray (2) and start ray (3).  The sides join interfaces of different knot counts, so each
+   centre has its own Gaussian width, shape x 3.5 x its local knot spacing, all of a side's widths
+   shrinking together until the side's Psi has cond below 1e11. */
 pdds_status pdds_mesh_create_disc(double radius, int32_t n_rings, int32_t n_sectors, uint32_t seed, int32_t kmin,
                                   int32_t kmax, double shape, pdds_mesh **out);
 /* stencil stopping domain of a knot: *geom 0 rectangle (domain[0..3] = x0, y0, x1, y1) or 1 annular
@@ -121,6 +123,9 @@ pdds_status pdds_mesh_geometry(const pdds_mesh *m, int32_t knot, int32_t *geom, 
 /* Gaussian RBF shape c of each stencil side of a knot (Compute_ipsi's fac[kind] on rectangular
    meshes, PDDAlgorithms/PDDSparseGM.cpp:816) */
 pdds_status pdds_mesh_shape(const pdds_mesh *m, int32_t knot, double c[4]);
+/* Gaussian width of every stencil entry of a knot, sides concatenated as in pdds_mesh_stencil
+   (disc meshes: one width per centre from the local knot spacing; rectangles: the side's c) */
+pdds_status pdds_mesh_entry_shapes(const pdds_mesh *m, int32_t knot, int32_t cap, double *c, int32_t *n_entries);
 void pdds_mesh_destroy(pdds_mesh *m);
 int32_t pdds_mesh_n_knots(const pdds_mesh *m);
 int32_t pdds_mesh_n_stencils(const pdds_mesh *m);      /* distinct stencils */

@@ -107,16 +107,17 @@ double orc_sector_dist(const double g[6], double x, double y, double *ex, double
     const double r0 = g[0], r1 = g[1], ca = g[2], sa = g[3], cb = g[4], sb = g[5];
     const double r = sqrt(x * x + y * y);
     const double sA = ca * y - sa * x, sB = x * sb - y * cb;
+    const int whole = r0 < 0.0;   /* the whole disc r <= r1: its circle is the only side */
     double d = r - r1;
     int k = 0;
     const double din = r0 > 0.0 ? r0 - r : -(r + r1);
     if (din > d) { d = din; k = 1; }
-    if (-sB > d) { d = -sB; k = 2; }
-    if (-sA > d) { d = -sA; k = 3; }
+    if (!whole && -sB > d) { d = -sB; k = 2; }
+    if (!whole && -sA > d) { d = -sA; k = 3; }
     if (k <= 1) {
         double ux = x / r, uy = y / r;
-        if (sA < 0.0) { ux = ca; uy = sa; }
-        else if (sB < 0.0) { ux = cb; uy = sb; }
+        if (!whole && sA < 0.0) { ux = ca; uy = sa; }
+        else if (!whole && sB < 0.0) { ux = cb; uy = sb; }
         const double rr = k == 0 ? r1 : r0;
         *ex = rr * ux; *ey = rr * uy;
         *nx = k == 0 ? ux : -ux; *ny = k == 0 ? uy : -uy;
@@ -396,6 +397,7 @@ static inline int orc_classify_geom(int geom, const double *dom, const double gl
     if (r >= glob[2] - 1e-8) return 4;
     if (dom[0] > 0.0 && orc_same(r, dom[0])) return 1;
     if (orc_same(r, dom[1])) return 0;
+    if (dom[0] < 0.0) return -1;
     if (fabs(dom[2] * y - dom[3] * x) < 1e-8) return 3;
     if (fabs(x * dom[5] - y * dom[4]) < 1e-8) return 2;
     return -1;
@@ -537,7 +539,7 @@ int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int6
                           int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                           double *eq_mean, double *eq_var) {
     return orc_reduce_batched_geom(pb, npaths, S, P, xt, yt, Y, Z, xi, 0, rect, glob, msz, idx, px, py, cshape,
-                                   ipsi, quirk, w, cols, G, Gvar, B, BB, cnt, eq_mean, eq_var);
+                                   ipsi, quirk, w, NULL, cols, G, Gvar, B, BB, cnt, eq_mean, eq_var);
 }
 
 /* As orc_reduce_batched_eq, with the exit sides classified on stopping domain dom of geometry
@@ -546,7 +548,7 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
                             const double *yt, const double *Y, const double *Z, const double *xi, int geom,
                             const double *rect, const double glob[4], const int msz[4],
                             const int *idx, const double *px, const double *py, const double cshape[4],
-                            const double *ipsi, int quirk, const double *w,
+                            const double *ipsi, int quirk, const double *w, const double *cent,
                             int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                             double *eq_mean, double *eq_var) {
     int off[5] = {0, msz[0], msz[0] + msz[1], msz[0] + msz[1] + msz[2], msz[0] + msz[1] + msz[2] + msz[3]};
@@ -597,7 +599,9 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
                         cnt[side]++;
                         for (int j = 0; j < msz[side]; ++j)
                             Phi[off[side] + j] = Phi[off[side] + j] +
-                                Y[n] * orc_rbf_ic2(px[off[side] + j], py[off[side] + j], xt[n], yt[n], ic2[side]);
+                                Y[n] * orc_rbf_ic2(px[off[side] + j], py[off[side] + j], xt[n], yt[n],
+                                                   cent ? 1.0 / (cent[off[side] + j] * cent[off[side] + j])
+                                                        : ic2[side]);
                     }
                     sc = Z[n] + xi[n];
                 }

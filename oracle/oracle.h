@@ -129,12 +129,13 @@ int orc_reduce_batched_eq(const orc_problem *pb, int64_t npaths, int64_t S, int6
                           const double *ipsi, int quirk, const double *w,
                           int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                           double *eq_mean, double *eq_var);
-/* orc_reduce_batched_eq on a stopping domain of geometry geom (disc radius in glob[2]) */
+/* orc_reduce_batched_eq on a stopping domain of geometry geom (disc radius in glob[2]); cent: per-entry
+   Gaussian widths (NULL: cshape of the entry's side) */
 int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, int64_t P, const double *xt,
                             const double *yt, const double *Y, const double *Z, const double *xi, int geom,
                             const double *rect, const double glob[4], const int msz[4],
                             const int *idx, const double *px, const double *py, const double cshape[4],
-                            const double *ipsi, int quirk, const double *w,
+                            const double *ipsi, int quirk, const double *w, const double *cent,
                             int *cols, double *G, double *Gvar, double *B, double *BB, int64_t cnt[6],
                             double *eq_mean, double *eq_var);
 

@@ -73,7 +73,7 @@ def lib():
         L.orc_reduce_batched_eq.argtypes = (red[:2] + [C.c_int64, C.c_int64] + red[2:16] + [_d] + red[16:] +
                                             [_d, _d])
         L.orc_reduce_batched_geom.argtypes = (red[:2] + [C.c_int64, C.c_int64] + red[2:7] + [C.c_int] + red[7:16] +
-                                              [_d] + red[16:] + [_d, _d])
+                                              [_d, _d] + red[16:] + [_d, _d])
         L.orc_bicgstab.argtypes = [C.c_int, _i64, _i, _d, _d, _d, C.c_double, C.c_int, _d]
         L.orc_philox.argtypes = [_u32, _u32, _u32]
         L.orc_math_eval.argtypes = [C.c_int, _d, _d, C.c_int64]
@@ -251,6 +251,9 @@ def _reduce(fn, pb, paths, st, glob, cshape, ipsi_list, quirk, S=None, P=None, w
         wv = np.zeros(tot + 1)
         wv[:len(w)] = w
         args += [ptr(wv)]
+    if geom is not None:
+        cent = st.get("cent")
+        args += [ptr(None) if cent is None or len(cent) == 0 else ptr(np.ascontiguousarray(cent, dtype=np.float64))]
     args += [ptr(cols, _i), ptr(G), ptr(Gv), ptr(B), ptr(BB), ptr(cnt, _i64)]
     if w is not None:
         args += [ptr(eq[0:1]), ptr(eq[1:2])]

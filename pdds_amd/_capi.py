@@ -111,6 +111,7 @@ EXPORTS = {
                                           C.c_double, C.POINTER(C.c_void_p)]),
     "pdds_mesh_geometry": (C.c_int32, [C.c_void_p, C.c_int32, _i32, _d]),
     "pdds_mesh_shape": (C.c_int32, [C.c_void_p, C.c_int32, _d]),
+    "pdds_mesh_entry_shapes": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _d, _i32]),
     "pdds_mesh_destroy": (None, [C.c_void_p]),
     "pdds_mesh_n_knots": (C.c_int32, [C.c_void_p]),
     "pdds_mesh_n_stencils": (C.c_int32, [C.c_void_p]),

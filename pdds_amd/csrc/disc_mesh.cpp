@@ -5,12 +5,14 @@
 //   - interfaces: the arcs of the inner circles between neighbouring rays, and the ray
 //     segments between neighbouring circles (the centre is no knot);
 //   - knots: n_I equispaced interior knots per interface (n_I uniform in [kmin, kmax] from a
-//     seeded hash), a knot at every crossing of an inner circle with a ray, and a global-boundary
-//     knot where each ray meets the outer circle;
+//     seeded hash), a knot at every crossing of an inner circle with a ray, one at the centre
+//     (where the rays meet), and a global-boundary knot where each ray meets the outer circle;
 //   - stencil of a knot: the union of the subdomains around its interface (an annular sector
 //     spanning two rings or two sectors, or both for a crossing), with four sides: outer arc (0),
 //     inner arc (1), end ray b (2) and start ray a (3), each listing the knots on it in order.
-//     An outer arc on the disc boundary is a global side; a ring-0 stencil has no inner side.
+//     An outer arc on the disc boundary is a global side; a ring-0 stencil has no inner side and
+//     its rays start at the centre knot.  The centre knot's stencil is the whole first disc,
+//     whose circle is its one side.
 #include "mesh.h"
 
 #include <cmath>
@@ -67,6 +69,7 @@ Mesh Mesh::disc(double R, int nr, int ns, uint32_t seed, int kmin, int kmax, dou
         }
     for (int i = 1; i < nr; ++i)
         for (int k = 0; k < ns; ++k) cross[(size_t)i * ns + k] = add(rho[i] * ca[k], rho[i] * sa[k], 0);
+    const int centre = add(0.0, 0.0, 0);
     for (int k = 0; k < ns; ++k) bnd[k] = add(R * ca[k], R * sa[k], 1);
     m.nknots_ = (int)X.size();
     m.x0 = X; m.y0 = Y; m.npx = X; m.npy = Y;
@@ -87,7 +90,7 @@ Mesh Mesh::disc(double R, int nr, int ns, uint32_t seed, int kmin, int kmax, dou
     auto ray_side = [&](int k, int i0, int i1) {
         std::vector<int> v;
         for (int i = i0; i < i1; ++i) {
-            if (i >= 1) v.push_back(cross[(size_t)i * ns + k]);
+            v.push_back(i >= 1 ? cross[(size_t)i * ns + k] : centre);
             const auto &a = ray[(size_t)i * ns + k];
             v.insert(v.end(), a.begin(), a.end());
         }
@@ -113,8 +116,20 @@ Mesh Mesh::disc(double R, int nr, int ns, uint32_t seed, int kmin, int kmax, dou
             s.kind[d] = 0;
             s.idx[d] = side[d];
             for (int v : side[d]) { s.px[d].push_back(X[v]); s.py[d].push_back(Y[v]); }
+            // per-centre widths from the local spacing along the side: the sides join interfaces
+            // of different knot counts, and one width for all would leave the sparse part
+            // under-resolved or the dense part singular
             const size_t n = side[d].size();
             s.cshape[d] = n > 1 ? shape * 3.5 * len[d] / (double)(n - 1) : 1.0;
+            if (n > 1) {
+                std::vector<double> gap(n - 1);
+                for (size_t j = 0; j + 1 < n; ++j) gap[j] = std::hypot(s.px[d][j + 1] - s.px[d][j], s.py[d][j + 1] - s.py[d][j]);
+                s.cent[d].resize(n);
+                for (size_t j = 0; j < n; ++j) {
+                    const double loc = j == 0 ? gap[0] : (j + 1 == n ? gap[n - 2] : 0.5 * (gap[j - 1] + gap[j]));
+                    s.cent[d][j] = shape * 3.5 * loc;
+                }
+            }
         }
         s.cond_max = 1e11;
         // bounding box: corners, plus the outer circle where it crosses an axis inside the sector
@@ -140,6 +155,35 @@ Mesh Mesh::disc(double R, int nr, int ns, uint32_t seed, int kmin, int kmax, dou
     for (int i = 0; i < nr; ++i)
         for (int k = 0; k < ns; ++k)
             for (int q : ray[(size_t)i * ns + k]) make(q, i, i + 1, wrap(k - 1), 2);
+    {
+        // the centre: the whole first disc, its circle (every crossing and arc knot on it, in
+        // angle order) the one side
+        Stencil s;
+        s.geo[0] = -1.0; s.geo[1] = rho[1]; s.geo[2] = 1.0; s.geo[3] = 0.0; s.geo[4] = 1.0; s.geo[5] = 0.0;
+        std::vector<int> side;
+        for (int k = 0; k < ns; ++k) {
+            side.push_back(cross[(size_t)ns + k]);
+            const auto &a = arc[(size_t)ns + k];
+            side.insert(side.end(), a.begin(), a.end());
+        }
+        const size_t n = side.size();
+        for (int d = 0; d < 4; ++d) s.kind[d] = 0;
+        s.idx[0] = side;
+        for (int v : side) { s.px[0].push_back(X[v]); s.py[0].push_back(Y[v]); }
+        s.cshape[0] = shape * 3.5 * (2.0 * PI * rho[1]) / (double)n;
+        s.cent[0].resize(n);
+        for (size_t j = 0; j < n; ++j) {
+            const size_t jp = (j + 1) % n, jm = (j + n - 1) % n;
+            const double g1 = std::hypot(s.px[0][jp] - s.px[0][j], s.py[0][jp] - s.py[0][j]);
+            const double g0 = std::hypot(s.px[0][j] - s.px[0][jm], s.py[0][j] - s.py[0][jm]);
+            s.cent[0][j] = shape * 3.5 * 0.5 * (g0 + g1);
+        }
+        s.cond_max = 1e11;
+        for (int i = 0; i < 4; ++i) s.rect[i] = 0.0;
+        s.bbox[0] = -rho[1]; s.bbox[1] = -rho[1]; s.bbox[2] = rho[1]; s.bbox[3] = rho[1];
+        per_knot[centre] = std::move(s);
+        has[centre] = 1;
+    }
     m.finalize(per_knot, has);
     return m;
 }

@@ -103,12 +103,14 @@ pdds_status dalloc_copy(pdds_engine *e, const std::vector<T> &v, T **out, size_t
 // Upload host stencil tables into T (device memory owned by T).
 pdds_status build_tables(pdds_engine *e, const std::vector<StencilDesc> &sten, const std::vector<double> &entx,
                          const std::vector<double> &enty, const std::vector<int8_t> &ents,
+                         const std::vector<double> &entic2,
                          const std::vector<int32_t> &e1, const std::vector<int32_t> &e2,
                          const std::vector<double> &ipsi, DevTables &T) {
     T.release();
     pdds_status st;
     if ((st = dalloc_copy(e, sten, &T.sten, T.bytes)) || (st = dalloc_copy(e, entx, &T.entx, T.bytes)) ||
         (st = dalloc_copy(e, enty, &T.enty, T.bytes)) || (st = dalloc_copy(e, ents, &T.ents, T.bytes)) ||
+        (st = dalloc_copy(e, entic2, &T.entic2, T.bytes)) ||
         (st = dalloc_copy(e, e1, &T.e1, T.bytes)) || (st = dalloc_copy(e, e2, &T.e2, T.bytes)) ||
         (st = dalloc_copy(e, ipsi, &T.ipsi, T.bytes))) {
         T.release();
@@ -197,8 +199,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     }
     std::vector<TableDesc> tdesc(tabs.size());
     for (size_t t = 0; t < tabs.size(); ++t)
-        tdesc[t] = TableDesc{tabs[t]->sten, tabs[t]->entx, tabs[t]->enty, tabs[t]->ents, tabs[t]->e1, tabs[t]->e2,
-                             tabs[t]->ipsi};
+        tdesc[t] = TableDesc{tabs[t]->sten, tabs[t]->entx, tabs[t]->enty, tabs[t]->ents, tabs[t]->entic2,
+                             tabs[t]->e1, tabs[t]->e2, tabs[t]->ipsi};
     TableDesc *dtab;
     if ((st = upload(e, B_TABLES, tdesc, &dtab))) return st;
     double kernel_ms = 0.0, reduce_ms = 0.0;
@@ -374,7 +376,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
 
 // Stencil tables of a mesh (device layout), built per quirk mode.
 void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::vector<double> &entx,
-                 std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<int32_t> &e1,
+                 std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<double> &entic2,
+                 std::vector<int32_t> &e1,
                  std::vector<int32_t> &e2, std::vector<double> &ipsi) {
     for (const Stencil &s : m.stencils) {
         StencilDesc d;
@@ -393,7 +396,13 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
         for (int i = 0; i < 4; ++i) {
             d.ipsi_off[i] = (int64_t)ipsi.size();
             ipsi.insert(ipsi.end(), s.ipsi[i].begin(), s.ipsi[i].end());
-            for (size_t j = 0; j < s.idx[i].size(); ++j) { entx.push_back(s.px[i][j]); enty.push_back(s.py[i][j]); ents.push_back((int8_t)i); }
+            for (size_t j = 0; j < s.idx[i].size(); ++j) {
+                entx.push_back(s.px[i][j]);
+                enty.push_back(s.py[i][j]);
+                ents.push_back((int8_t)i);
+                const double c = s.cent[i].empty() ? s.cshape[i] : s.cent[i][j];
+                entic2.push_back(1.0 / (c * c));
+            }
         }
         const auto &a1 = quirk ? s.e1 : s.e1_sum;
         for (size_t c = 0; c < s.cols.size(); ++c) {
@@ -405,18 +414,21 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
 }
 
 // Device-resident stencil tables of (mesh, quirk): built on first use, then reused.
+// Disc meshes always sum a corner entry's sides: the "=+" quirk is a property of the
+// reference's rectangular stencils, and on a disc the apex of every centre sector is such a
+// corner, reached by most of the paths of the knots next to it.
 pdds_status mesh_dev_tables(pdds_engine *e, const pdds_mesh *m, int quirk, DevTables **out) {
-    const int q = quirk ? 1 : 0;
+    const int q = (quirk && m->m.geom == GeomRect) ? 1 : 0;
     for (DevTables &t : e->tables)
         if (t.mesh_id == m->id && t.quirk == q) { *out = &t; return PDDS_OK; }
     std::vector<StencilDesc> sten;
-    std::vector<double> entx, enty, ipsi;
+    std::vector<double> entx, enty, ipsi, entic2;
     std::vector<int8_t> ents;
     std::vector<int32_t> e1, e2;
-    mesh_tables(m->m, q, sten, entx, enty, ents, e1, e2, ipsi);
+    mesh_tables(m->m, q, sten, entx, enty, ents, entic2, e1, e2, ipsi);
     DevTables &t = e->tables[e->tables_next];
     e->tables_next = (e->tables_next + 1) % 2;
-    pdds_status st = build_tables(e, sten, entx, enty, ents, e1, e2, ipsi, t);
+    pdds_status st = build_tables(e, sten, entx, enty, ents, entic2, e1, e2, ipsi, t);
     if (st) return st;
     t.mesh_id = m->id;
     t.quirk = q;
@@ -662,6 +674,22 @@ pdds_status pdds_mesh_geometry(const pdds_mesh *m, int32_t knot, int32_t *geom, 
     return PDDS_OK;
 }
 
+pdds_status pdds_mesh_entry_shapes(const pdds_mesh *m, int32_t knot, int32_t cap, double *c, int32_t *n_entries) {
+    if (!m || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
+    const int t = m->m.stencil_of[knot];
+    int tot = t >= 0 ? m->m.stencils[t].total() : 0;
+    if (n_entries) *n_entries = tot;
+    if (c) {
+        if (cap < tot) return PDDS_ERR_INVALID;
+        int o = 0;
+        for (int d = 0; t >= 0 && d < 4; ++d) {
+            const Stencil &s = m->m.stencils[t];
+            for (size_t j = 0; j < s.idx[d].size(); ++j) c[o++] = s.cent[d].empty() ? s.cshape[d] : s.cent[d][j];
+        }
+    }
+    return PDDS_OK;
+}
+
 pdds_status pdds_mesh_shape(const pdds_mesh *m, int32_t knot, double c[4]) {
     if (!m || !c || knot < 0 || knot >= m->m.n_knots()) return PDDS_ERR_INVALID;
     const int t = m->m.stencil_of[knot];
@@ -835,11 +863,11 @@ void pdds_adaptive_defaults(pdds_adaptive *c) {
     c->growth = 1.1;
 }
 
-// Pilot assembly and solve, then rounds of: assemble with the current counts and the previous
-// solution as reference, solve, and raise each knot whose equation standard error
-// sqrt(var_k / N_k) is above the target to N_k = growth * var_k / se^2 (within [n_min, n_max]).
-// Counts only grow; every round re-runs paths [0, N_k) of each knot, so the final G, B and u
-// are those of one plain assembly with the returned counts.  The policy stands in for the
+// Pilot assembly and solve, then rounds of: assemble with the current counts, solve, and move
+// each knot's count towards N_k = growth * var_k / se^2 (within [n_min, n_max]), var_k the
+// batch-means variance of its equation sqrt(var_k / N_k) being its standard error.  Every round
+// re-runs paths [0, N_k) of each knot, so the final G, B and u are those of one plain assembly
+// with the returned counts.  The policy stands in for the
 // commented Compute_h_N of the reference (PDDAlgorithms/PDDSparseGM.cpp:4338-4395, N_i =
 // 4 Var_i (1 - rho_i^2) / (0.6667 eps)^2) with the batch-means variance of the knot equation,
 // control variate included, in place of its Var_i (1 - rho_i^2).
@@ -880,29 +908,50 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
     auto solve = [&](double *out) -> pdds_status {
         return solve_system(e, n, M.row_ptr.data(), M.cols.data(), G.data(), B.data(), out, tol, 0, &it, &rr);
     };
-    // pilot: a solution to weigh the equations with, then their variances at n_pilot paths
+    // The equations are weighed with the most accurate solution so far (the one from the
+    // assembly with the most paths): a noisy reference inflates the measured variance through
+    // the interpolation of its noise along the stencil sides.  Counts move towards
+    // growth * var / se^2 by at most a factor 8 per round, up or down, so that an over-estimate
+    // from an early, noisy reference is taken back once the reference improves.
     pdds_status st;
     if ((st = assemble(nullptr)) || (st = solve(uref.data())) || (st = assemble(uref.data()))) return st;
+    int64_t best_paths = 0;
+    for (int q = 0; q < n; ++q)
+        if (!M.on_boundary[q]) best_paths += N[q];
     auto target = [&](int q) -> int64_t {
         const double want = std::ceil(cfg.growth * ev[q] / (cfg.se_target * cfg.se_target));
-        const double c = std::min(std::max(want, (double)cfg.n_min), (double)cfg.n_max);
+        double c = std::min(std::max(want, (double)cfg.n_min), (double)cfg.n_max);
+        c = std::min(c, 8.0 * (double)N[q]);
+        c = std::max(c, std::max((double)cfg.n_min, std::floor((double)N[q] / 8.0)));
         return (int64_t)c;
     };
     for (int q = 0; q < n; ++q)
-        if (!M.on_boundary[q]) N[q] = std::max(N[q], target(q));
+        if (!M.on_boundary[q]) N[q] = target(q);
     for (int round = 0; round < cfg.max_rounds; ++round) {
         if ((st = assemble(uref.data())) || (st = solve(u))) return st;
         R.rounds = round + 1;
-        std::vector<int> raise;
+        bool over = false, moved = false;
+        std::vector<int64_t> next(N);
         for (int q = 0; q < n; ++q) {
             if (M.on_boundary[q]) continue;
             const double sek = std::sqrt(std::max(ev[q], 0.0) / (double)N[q]);
-            if (sek > cfg.se_target && N[q] < cfg.n_max) raise.push_back(q);
+            if (sek > cfg.se_target && N[q] < cfg.n_max) over = true;
+            next[q] = target(q);
+            // a knot within target keeps its count unless the target is well below it
+            if (sek <= cfg.se_target && next[q] > N[q]) next[q] = N[q];
+            if (sek <= cfg.se_target && next[q] * 2 > N[q]) next[q] = N[q];
+            moved = moved || next[q] != N[q];
         }
         // the counts returned are those of the last assembly
-        if (raise.empty() || round + 1 == cfg.max_rounds) break;
-        for (int q : raise) N[q] = std::max(N[q] + 1, target(q));
-        std::copy(u, u + n, uref.begin());
+        if ((!over && !moved) || round + 1 == cfg.max_rounds) break;
+        int64_t tot = 0;
+        for (int q = 0; q < n; ++q)
+            if (!M.on_boundary[q]) tot += N[q];
+        if (tot >= best_paths) {
+            best_paths = tot;
+            std::copy(u, u + n, uref.begin());
+        }
+        N.swap(next);
     }
     R.paths = 0;
     R.max_se = 0.0;
@@ -1066,7 +1115,8 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
     std::vector<int8_t> ents;
     std::vector<int32_t> e1, e2;
     DevTables T;
-    if ((st = build_tables(e, sten, entx, enty, ents, e1, e2, ipsi, T))) return st;
+    std::vector<double> entic2;
+    if ((st = build_tables(e, sten, entx, enty, ents, entic2, e1, e2, ipsi, T))) return st;
     struct Rel { DevTables &t; ~Rel() { t.release(); } } rel{T};
     std::vector<AsmKnot> knots;
     std::vector<int> kmap;

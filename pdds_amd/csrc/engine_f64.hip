@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         for (int e = tid; e < ne; e += WG) {
             const int se = tb.ents[sd.ent_off + e];
             const double pxe = tb.entx[sd.ent_off + e], pye = tb.enty[sd.ent_off + e];
-            const double ic2 = sd.inv_c2[se];
+            const double ic2 = tb.entic2[sd.ent_off + e];
             for (int b = 0; b < nb; ++b) {
                 const uint16_t *L = lst + (b * 4 + se) * S;
                 const int nl = s_lcnt[b * 4 + se];

@@ -27,16 +27,16 @@ struct DevTables {
     int geom = 0;                         // GeomRect / GeomDisc stopping domains
     int n_sten = 0;
     StencilDesc *sten = nullptr;
-    double *entx = nullptr, *enty = nullptr, *ipsi = nullptr;
+    double *entx = nullptr, *enty = nullptr, *ipsi = nullptr, *entic2 = nullptr;
     int8_t *ents = nullptr;
     int32_t *e1 = nullptr, *e2 = nullptr;
     std::vector<StencilDesc> host_sten;   // sizes for launch planning
     size_t bytes = 0;
     void release() {
-        void *ps[] = {sten, entx, enty, ipsi, ents, e1, e2};
+        void *ps[] = {sten, entx, enty, ipsi, entic2, ents, e1, e2};
         for (void *p : ps)
             if (p) (void)hipFree(p);
-        sten = nullptr; entx = enty = ipsi = nullptr; ents = nullptr; e1 = e2 = nullptr;
+        sten = nullptr; entx = enty = ipsi = entic2 = nullptr; ents = nullptr; e1 = e2 = nullptr;
         mesh_id = 0; quirk = -1; geom = 0; n_sten = 0; host_sten.clear(); bytes = 0;
     }
 };
@@ -131,6 +131,7 @@ struct AsmOut {
 int choose_P(int64_t n, int S);
 pdds_status build_tables(pdds_engine *e, const std::vector<StencilDesc> &sten, const std::vector<double> &entx,
                          const std::vector<double> &enty, const std::vector<int8_t> &ents,
+                         const std::vector<double> &entic2,
                          const std::vector<int32_t> &e1, const std::vector<int32_t> &e2,
                          const std::vector<double> &ipsi, DevTables &T);
 // W_host (optional, n_vals values): per CSR value the reference interface value of its column's
@@ -140,7 +141,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
                          int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out,
                          const double *W_host = nullptr);
 void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::vector<double> &entx,
-                 std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<int32_t> &e1,
+                 std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<double> &entic2,
+                 std::vector<int32_t> &e1,
                  std::vector<int32_t> &e2, std::vector<double> &ipsi);
 pdds_status mesh_dev_tables(pdds_engine *e, const pdds_mesh *m, int quirk, DevTables **out);
 void fill_counters(pdds_counters *ctr, const AsmOut &o, double total_ms);

@@ -59,6 +59,7 @@ PDDS_HD int classify_sector(const double g[6], double R, double x, double y) {
     if (r >= R - 1e-8) return 4;
     if (g[0] > 0.0 && __builtin_fabs(r - g[0]) < 1e-8) return 1;
     if (__builtin_fabs(r - g[1]) < 1e-8) return 0;
+    if (g[0] < 0.0) return -1;   // the whole disc r <= r1 has no other side
     if (__builtin_fabs(g[2] * y - g[3] * x) < 1e-8) return 3;
     if (__builtin_fabs(x * g[5] - y * g[4]) < 1e-8) return 2;
     return -1;

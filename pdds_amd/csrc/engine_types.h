@@ -56,6 +56,7 @@ struct TableDesc {
     const StencilDesc *sten;
     const double *entx, *enty;     // side-entry positions
     const int8_t *ents;            // side of each entry
+    const double *entic2;          // 1 / c^2 of each entry's Gaussian (its side's c, or its own)
     const int32_t *e1, *e2;        // contributing entries of each CSR column
     const double *ipsi;            // Psi^-1 blocks
 };

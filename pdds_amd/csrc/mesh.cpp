@@ -195,10 +195,12 @@ void Mesh::finalize(std::vector<Stencil> &per_knot, const std::vector<char> &has
         for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
             Stencil &s = stencils[jobs[j].first];
             const int d = jobs[j].second;
-            bool good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]);
+            const std::vector<double> *cw = s.cent[d].empty() ? nullptr : &s.cent[d];
+            bool good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d], cw);
             for (int t = 0; s.cond_max > 0.0 && t < 40 && (!good || s.cond[d] > s.cond_max); ++t) {
                 s.cshape[d] *= 0.8;
-                good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d]);
+                for (double &v : s.cent[d]) v *= 0.8;
+                good = psi_inverse(s.px[d], s.py[d], s.cshape[d], s.ipsi[d], s.cond[d], cw);
             }
             ok[j] = good && !(s.cond_max > 0.0 && s.cond[d] > s.cond_max) ? 1 : 0;
         }
@@ -325,14 +327,15 @@ Stencil Mesh::compute_stencil(int k) const {
 }
 
 bool psi_inverse(const std::vector<double> &px, const std::vector<double> &py, double c, std::vector<double> &ipsi,
-                 double &cond) {
+                 double &cond, const std::vector<double> *cw) {
     const int m = (int)px.size();
     std::vector<double> A((size_t)m * m), LU;
     for (int i = 0; i < m; ++i)
         for (int j = 0; j < m; ++j) {
             double dx = px[i] - px[j], dy = py[i] - py[j];
             double r = std::sqrt(dx * dx + dy * dy);
-            A[(size_t)i * m + j] = dexp(-(r * r) / (c * c));
+            const double cj = cw ? (*cw)[j] : c;
+            A[(size_t)i * m + j] = dexp(-(r * r) / (cj * cj));
         }
     LU = A;
     std::vector<int> pr(m), pc(m);

@@ -46,6 +46,7 @@ struct Stencil {
     std::vector<double> px[4], py[4];
     // derived
     double cshape[4] = {1, 1, 1, 1};
+    std::vector<double> cent[4];           // per-entry Gaussian widths (empty: cshape of the side)
     double cond[4] = {0, 0, 0, 0};
     std::vector<double> ipsi[4];          // row-major m x m
     std::vector<int> cols;                 // sorted unique CSR columns
@@ -93,7 +94,8 @@ class Mesh {
 };
 
 // Full-pivot LU inverse of the Gaussian RBF matrix; returns false if singular.
+// With cw, centre j has its own width cw[j]: Psi_ij = exp(-|p_i - p_j|^2 / cw[j]^2).
 bool psi_inverse(const std::vector<double> &px, const std::vector<double> &py, double c, std::vector<double> &ipsi,
-                 double &cond);
+                 double &cond, const std::vector<double> *cw = nullptr);
 
 }  // namespace pdds

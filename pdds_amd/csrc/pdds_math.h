@@ -360,7 +360,8 @@ PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kin
 
 // Signed distance to the annular sector g = {r0, r1, cos a, sin a, cos b, sin b}: radii
 // r0 <= r <= r1 between the rays a and b (b counterclockwise from a, less than pi apart;
-// r0 = 0 for a sector at the centre, which then has no inner side).  Inside, the distance to
+// r0 = 0 for a sector at the centre, which then has no inner side; r0 < 0 for the whole disc
+// r <= r1, whose only side is its circle).  Inside, the distance to
 // the nearest side is the largest of the four half-plane / annulus distances (exact for such
 // a sector); outside, that largest violation stands for the distance (positive).  The side it
 // names gives the projection (clamped to the side) and the outward normal; `kind` reports it
@@ -371,17 +372,18 @@ PDDS_HD R sector_dist(const R g[6], R x, R y, R &ex, R &ey, R &nx, R &ny, int &k
     const R r = RM<R>::sqrt(x * x + y * y);
     const R sA = ca * y - sa * x;          // > 0: counterclockwise of ray a (inside)
     const R sB = x * sb - y * cb;          // > 0: clockwise of ray b (inside)
+    const bool whole = r0 < (R)0;
     R d = r - r1;
     int k = 0;
     const R din = r0 > (R)0 ? r0 - r : -(r + r1);
     if (din > d) { d = din; k = 1; }
-    if (-sB > d) { d = -sB; k = 2; }
-    if (-sA > d) { d = -sA; k = 3; }
+    if (!whole && -sB > d) { d = -sB; k = 2; }
+    if (!whole && -sA > d) { d = -sA; k = 3; }
     if (k <= 1) {
         // radial direction of X, or the nearer ray's when X lies outside the angle
         R ux = x / r, uy = y / r;
-        if (sA < (R)0) { ux = ca; uy = sa; }
-        else if (sB < (R)0) { ux = cb; uy = sb; }
+        if (!whole && sA < (R)0) { ux = ca; uy = sa; }
+        else if (!whole && sB < (R)0) { ux = cb; uy = sb; }
         const R rr = k == 0 ? r1 : r0;
         ex = rr * ux; ey = rr * uy;
         nx = k == 0 ? ux : -ux; ny = k == 0 ? uy : -uy;

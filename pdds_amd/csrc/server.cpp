@@ -41,7 +41,7 @@ struct SrvReq {
     // solve
     std::unique_ptr<pdds_mesh> mesh;
     std::vector<StencilDesc> sten;    // host stencil tables (uploaded on admission)
-    std::vector<double> entx, enty, ipsi;
+    std::vector<double> entx, enty, ipsi, entic2;
     std::vector<int8_t> ents;
     std::vector<int32_t> e1, e2;
     DevTables tab;
@@ -99,7 +99,7 @@ void prepare(pdds_server *s, SrvReq *r) {
         const Mesh &M = r->mesh->m;
         if (M.any_singular) st = PDDS_ERR_SINGULAR;
         if (!st) {
-            mesh_tables(M, s->cfg.run.quirk_last_side, r->sten, r->entx, r->enty, r->ents, r->e1, r->e2, r->ipsi);
+            mesh_tables(M, s->cfg.run.quirk_last_side, r->sten, r->entx, r->enty, r->ents, r->entic2, r->e1, r->e2, r->ipsi);
             const int n = M.n_knots();
             r->G.assign(std::max<int64_t>(M.row_ptr.back(), 1), 0.0);
             r->B.assign(n, 0.0);
@@ -170,7 +170,7 @@ bool run_round(pdds_server *s) {
     std::vector<SrvReq *> failed;
     for (SrvReq *r : adm)
         if (r->rq.kind == PDDS_REQ_SOLVE && r->tab.sten == nullptr &&
-            build_tables(e, r->sten, r->entx, r->enty, r->ents, r->e1, r->e2, r->ipsi, r->tab) != PDDS_OK)
+            build_tables(e, r->sten, r->entx, r->enty, r->ents, r->entic2, r->e1, r->e2, r->ipsi, r->tab) != PDDS_OK)
             failed.push_back(r);
     if (!s->dom_ready) {
         StencilDesc d;
@@ -181,7 +181,7 @@ bool run_round(pdds_server *s) {
         std::vector<double> none;
         std::vector<int8_t> none8;
         std::vector<int32_t> none32;
-        if (build_tables(e, sd, none, none, none8, none32, none32, none, s->dom) == PDDS_OK) s->dom_ready = true;
+        if (build_tables(e, sd, none, none, none8, none, none32, none32, none, s->dom) == PDDS_OK) s->dom_ready = true;
     }
     // the round's knots: point paths first, then the next knots of each solve
     std::vector<const DevTables *> tabs = {&s->dom};
@@ -271,7 +271,7 @@ bool run_round(pdds_server *s) {
     ++s->n_rounds;
     for (SrvReq *r : complete) {
         r->tab.release();
-        r->sten.clear(); r->entx.clear(); r->enty.clear(); r->ipsi.clear(); r->ents.clear(); r->e1.clear(); r->e2.clear();
+        r->sten.clear(); r->entx.clear(); r->enty.clear(); r->ipsi.clear(); r->entic2.clear(); r->ents.clear(); r->e1.clear(); r->e2.clear();
         s->inflight_bytes -= r->footprint;
         s->admitted.erase(std::find(s->admitted.begin(), s->admitted.end(), r));
         finish(s, r, r->res.status);

@@ -313,8 +313,10 @@ class Mesh:
         A.lib().pdds_mesh_geometry(self.h, k, None, ptr(dom))
         cshape = np.zeros(4)
         A.lib().pdds_mesh_shape(self.h, k, ptr(cshape))
+        cent = np.zeros(max(t, 1))
+        A.lib().pdds_mesh_entry_shapes(self.h, k, len(cent), ptr(cent), None)
         return {"rect": rect, "msz": msz, "kind": kind, "idx": idx[:t], "px": px[:t], "py": py[:t], "cond": cond,
-                "ipsi": ipsi, "domain": dom, "geom": self.geom, "cshape": cshape}
+                "ipsi": ipsi, "domain": dom, "geom": self.geom, "cshape": cshape, "cent": cent[:t]}
 
 
 class Server:

@@ -64,3 +64,21 @@ def build(name, engine=None, shape=None):
     else:
         prob = bvp(pde)
     return mesh, prob, n, h, seed
+
+
+# Batching configuration: the cfg2 PDE (divergence form, sigma/b/c from 512^2 LUTs) on a disc of
+# radius 1 cut into 6 rings x 24 sectors with 8..64 knots per interface (seed 11); per-knot path
+# counts from 1e3 to 1e6 targeting a standard error of 1e-3 per knot equation; FP64 paths.
+DISC = {"radius": 1.0, "n_rings": 6, "n_sectors": 24, "seed": 11, "kmin": 8, "kmax": 64, "shape": 1.0,
+        "h": 1e-4, "path_seed": 7, "se_target": 1e-3, "n_min": 1000, "n_max": 1000000}
+
+
+def build_disc(engine):
+    """Return (mesh, bvp, h, seed, adaptive config) of the batching configuration."""
+    c = DISC
+    mesh = Mesh.disc(c["radius"], c["n_rings"], c["n_sectors"], c["seed"], c["kmin"], c["kmax"], c["shape"])
+    r = c["radius"]
+    prob = bvp("varcoef_lut", lut=varcoef_lut(engine, 512, (-r, -r, r, r)))
+    adaptive = {"se_target": c["se_target"], "n_min": c["n_min"], "n_max": c["n_max"], "max_rounds": 6}
+    return mesh, prob, c["h"], c["path_seed"], adaptive
+

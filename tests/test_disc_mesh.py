@@ -25,8 +25,9 @@ def test_disc_counts_and_positions(disc):
     r = np.hypot(m.x0, m.y0)
     assert np.all(r <= 1.0 + 1e-12)
     assert m.on_boundary.sum() == 24 and np.allclose(r[m.on_boundary], 1.0)
-    # 120 arcs + 144 ray segments of 8..64 interior knots, 120 crossings, 24 rim knots
-    n_int = m.n_knots - 120 - 24
+    # 120 arcs + 144 ray segments of 8..64 interior knots, 120 crossings, the centre, 24 rim knots
+    n_int = m.n_knots - 120 - 1 - 24
+    assert np.sum((m.x0 == 0) & (m.y0 == 0)) == 1
     assert 264 * 8 <= n_int <= 264 * 64
     # crossings sit on the inner circles at the ray angles
     rings = np.round(r * 6.0)
@@ -39,13 +40,18 @@ def test_disc_counts_and_positions(disc):
 
 def test_disc_stencils_are_sectors_around_their_knot(disc):
     m = disc
-    for k in list(range(0, m.n_knots, 97)) + [m.n_knots - 25, m.n_knots - 30]:
+    centre = int(np.flatnonzero((m.x0 == 0) & (m.y0 == 0))[0])
+    for k in list(range(0, m.n_knots, 97)) + [m.n_knots - 25, m.n_knots - 30, centre]:
         st = m.stencil(k)
         if m.on_boundary[k]:
             assert st["msz"].sum() == 0
             continue
         g = st["domain"]
         r0, r1, ca, sa, cb, sb = g
+        if r0 < 0:   # the centre knot: the whole first disc, its circle the one side
+            assert m.x0[k] == 0 and m.y0[k] == 0 and st["msz"][1:].sum() == 0
+            assert np.allclose(np.hypot(st["px"], st["py"]), r1, atol=1e-12)
+            continue
         assert st["geom"] == 1 and r1 > r0 >= 0.0
         # the knot is strictly inside its stopping domain
         assert O.sector_dist(g, m.x0[k], m.y0[k])[0] < -1e-6
@@ -104,6 +110,12 @@ def test_sector_dist_closed_forms():
     # a centre sector (r0 = 0) has no inner side
     g0 = np.array([0.0, 0.6, np.cos(a), np.sin(a), np.cos(b), np.sin(b)])
     assert O.sector_dist(g0, 0.01 * np.cos(0.25), 0.01 * np.sin(0.25))[5] in (2, 3)
+    # the whole disc (r0 < 0): only its circle, whatever the angle
+    gw = np.array([-1.0, 0.5, 1.0, 0.0, 1.0, 0.0])
+    for t in (0.0, 2.0, -2.5):
+        dd, ex, ey, nx, ny, kd = O.sector_dist(gw, 0.2 * np.cos(t), 0.2 * np.sin(t))
+        assert kd == 0 and dd == pytest.approx(-0.3, abs=1e-15)
+        assert (ex, ey) == pytest.approx((0.5 * np.cos(t), 0.5 * np.sin(t)), abs=1e-15)
 
 
 def test_feynman_kac_on_a_sector_converges_to_u(disc):
@@ -127,3 +139,38 @@ def test_feynman_kac_on_a_sector_converges_to_u(disc):
         # exits land on the stopping domain's boundary
         d = np.array([O.sector_dist(st["domain"], a, b)[0] for a, b in zip(p["xt"][:500], p["yt"][:500])])
         assert np.all(np.abs(d) < 1e-12)
+
+
+def test_disc_side_interpolation_is_accurate(disc):
+    """Each side's RBF interpolant of a smooth u (the cardinal functions H of the G rows) is
+    accurate between its end knots, including sides that join interfaces of very different knot
+    counts (per-centre widths; a single width per side left errors near 0.2 there).  A ray side
+    from the centre has no knot at the centre itself, so it is checked from its first knot."""
+    m = disc
+    u = lambda x, y: np.exp(x) * np.cos(y)   # noqa: E731
+    worst = 0.0
+    for k in range(0, m.n_knots, 41):
+        if m.on_boundary[k]:
+            continue
+        st = m.stencil(k)
+        g = st["domain"]
+        off = np.concatenate([[0], np.cumsum(st["msz"])])
+        for s in range(4):
+            sl = slice(off[s], off[s + 1])
+            px, py, c = st["px"][sl], st["py"][sl], st["cent"][sl]
+            if len(px) < 2:
+                continue
+            if s <= 1:
+                r = g[1] if s == 0 else g[0]
+                t0, t1 = np.arctan2(g[3], g[2]), np.arctan2(g[5], g[4])
+                t1 = t1 + 2 * np.pi if t1 < t0 else t1
+                t = np.linspace(t0, t1, 300)
+                mx, my = r * np.cos(t), r * np.sin(t)
+            else:
+                ca, sa = (g[4], g[5]) if s == 2 else (g[2], g[3])
+                rr = np.linspace(np.hypot(px[0], py[0]), g[1], 300)
+                mx, my = rr * ca, rr * sa
+            phi = np.exp(-((px[None, :] - mx[:, None]) ** 2 + (py[None, :] - my[:, None]) ** 2) / c[None, :] ** 2)
+            interp = (phi @ st["ipsi"][s]) @ u(px, py)
+            worst = max(worst, np.abs(interp - u(mx, my)).max())
+    assert worst < 1e-2, worst

@@ -58,7 +58,8 @@ def test_disc_solve_knot_bit_exact(eng, disc, pde, plain, lut_lds):
         st = m.stencil(k)
         got = eng.solve_knot(m, prob, run, k, n)
         paths = O.simulate(pb, O.run_params(h, 19, threads=8), m.x0[k], m.y0[k], st["domain"], k, 0, n, geom=1)
-        ref = O.reduce_batched_eq(pb, paths, st, m.glob, st["cshape"], st["ipsi"], 32, 640, None, quirk=1, geom=1)
+        # disc meshes sum a corner entry's sides whatever quirk_last_side says
+        ref = O.reduce_batched_eq(pb, paths, st, m.glob, st["cshape"], st["ipsi"], 32, 640, None, quirk=0, geom=1)
         assert np.array_equal(got["cols"], ref["cols"]), k
         assert np.array_equal(got["G"], ref["G"]), k
         assert np.array_equal(got["Gvar"], ref["Gvar"]), k
@@ -70,8 +71,7 @@ def test_disc_solve_knot_bit_exact(eng, disc, pde, plain, lut_lds):
 def test_disc_adaptive_solve(eng):
     """Variance-targeted counts on a disc meet the target; each knot equation, weighed with the
     exact u, reproduces u at the knot within its error; the solved interface values are close to
-    u (the thin pie-slice stencils near the centre make I + G less well conditioned than on
-    rectangles, so the solution error exceeds the per-equation error)."""
+    u."""
     from pdds_amd.engine import Mesh, bvp, default_run
     m = Mesh.disc(1.0, 4, 12, 11, 8, 16, 1.0)
     prob = bvp("sinsin")
@@ -90,4 +90,4 @@ def test_disc_adaptive_solve(eng):
     err = out["u"] - uex
     rms = np.sqrt(np.mean(err ** 2))
     print(rep, rms, flush=True)
-    assert rms < 0.1, rms
+    assert rms < 2e-2, rms

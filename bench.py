@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 # counted separately and are not in this figure.
 FLOPS_PER_STEP = {"cfg2_varcoef_lut": 302.0, "cfg1_poisson": 41.0, "fixture": 41.0}
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 
 
 def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
@@ -232,6 +233,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2_varcoef_lut")
     ap.add_argument("--paths", type=int, default=0, help="override paths per knot (0: the config's)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="path precision of the timed assemblies (the configuration's is FP64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -268,7 +271,7 @@ def main():
     mesh, prob, n_paths, h, seed = W.build(args.config, eng)
     if args.paths:
         n_paths = args.paths
-    run = default_run(h=h, seed=seed)
+    run = default_run(h=h, seed=seed, precision=args.precision)
 
     for _ in range(args.warmup):
         eng.assemble(mesh, prob, run, n_paths)
@@ -289,8 +292,9 @@ def main():
     avg_kernel_s = float(np.mean(kern_ms)) / 1e3
     flops = FLOPS_PER_STEP.get(args.config, 302.0) * c["path_steps"]
     achieved = flops / avg_kernel_s / 1e12
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
     traffic = None
-    if os.path.exists(args.traffic_file):
+    if args.precision == "fp64" and os.path.exists(args.traffic_file):
         try:
             traffic = json.load(open(args.traffic_file)).get("bytes_per_launch")
         except (OSError, ValueError):
@@ -306,7 +310,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 sums)",
         "data": "synthetic (manufactured solution u = e^x cos y; LUTs sampled from the analytic coefficients)",
         "config": {"workload": "plain PDDSparse assembly, cfg2: 8x8 subdomains x 32 knots/interface, "
                                "3437 knots (3409 MC), %d paths/knot, h=%g, FP64 paths, 512x512 LUTs, seed %d"
@@ -320,15 +324,17 @@ def main():
         "launch": {"grid_wgs": c["grid_wgs"], "wg_size": c["wg_size"], "item_paths": c["item_paths"],
                    "sub_batch": c["sub_batch"], "work_items": c["work_items"]},
         "device": info,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": ("path_kernel_tiled<double> (LDS-tiled LUT)" if c["wg_size"] != 256
-                                else "path_kernel<double, PDE_VARCOEF_LUT, persistent>"),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": ("path_kernel_tiled<%s> (LDS-tiled LUT)" % ("double" if args.precision == "fp64" else "float")
+                                if c["wg_size"] != 256 else "path_kernel (persistent, LUT from global memory)"),
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
-                     "note": "no dense contraction: peak = MI355X FP64 vector (= FP64 matrix) 78.6 TFLOP/s; "
-                             "achieved counts %g algorithmic flops per path-step; traffic = HBM bytes per launch "
-                             "from FETCH_SIZE/WRITE_SIZE (copy-calibrated), mostly scattered exit-record stores"
-                             % FLOPS_PER_STEP.get(args.config, 302.0)},
+                     "note": ("no dense contraction: peak = MI355X %s; achieved counts %g algorithmic flops per "
+                              "path-step; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE (copy-calibrated), "
+                              "mostly scattered exit-record stores"
+                              % ("FP64 vector (= FP64 matrix) 78.6 TFLOP/s" if args.precision == "fp64"
+                                 else "FP32 vector 157.3 TFLOP/s (packed; this build issues unpacked FP32)",
+                                 FLOPS_PER_STEP.get(args.config, 302.0)))},
     }
     if not args.no_extras:
         extras = {}

@@ -31,6 +31,7 @@ void orc_math_eval(int fn, const double *x, double *y, int64_t n) {
             case 4: y[i] = orc_tanh(x[i]); break;
             case 5: y[i] = orc_sinpi(x[i]); break;
             case 6: y[i] = orc_cospi(x[i]); break;
+            case 8: y[i] = orc_exp_tab(x[i]); break;
             default: y[i] = orc_expm1(x[i]); break;
         }
     }
@@ -413,7 +414,7 @@ static inline double orc_rbf(double px, double py, double x, double y, double c)
 /* The same kernel as the GPU engine evaluates it: squared distance times a precomputed 1/c^2 */
 static inline double orc_rbf_ic2(double px, double py, double x, double y, double inv_c2) {
     double dx = px - x, dy = py - y;
-    return orc_exp(-(dx * dx + dy * dy) * inv_c2);
+    return orc_exp_tab(-(dx * dx + dy * dy) * inv_c2);
 }
 
 /* CSR row of a stencil: sorted unique columns; for each side entry its column slot. */
@@ -503,7 +504,7 @@ static double orc_wg_sum(const double *G, const double *w, int nnz) {
     double part[256];
     for (int t = 0; t < 256; ++t) {
         part[t] = 0.0;
-        for (int k = t; k < nnz; k += 256) part[t] = part[t] + G[k] * w[k];
+        for (int k = t; k < nnz; k += 256) part[t] = fma(G[k], w[k], part[t]);
     }
     double T = 0.0;
     for (int wv = 0; wv < 4; ++wv) {
@@ -598,15 +599,15 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
                     else {
                         cnt[side]++;
                         for (int j = 0; j < msz[side]; ++j)
-                            Phi[off[side] + j] = Phi[off[side] + j] +
-                                Y[n] * orc_rbf_ic2(px[off[side] + j], py[off[side] + j], xt[n], yt[n],
-                                                   cent ? 1.0 / (cent[off[side] + j] * cent[off[side] + j])
-                                                        : ic2[side]);
+                            Phi[off[side] + j] = fma(Y[n],
+                                orc_rbf_ic2(px[off[side] + j], py[off[side] + j], xt[n], yt[n],
+                                            cent ? 1.0 / (cent[off[side] + j] * cent[off[side] + j]) : ic2[side]),
+                                Phi[off[side] + j]);
                     }
                     sc = Z[n] + xi[n];
                 }
                 sb = sb + sc;
-                sbb = sbb + sc * sc;
+                sbb = fma(sc, sc, sbb);
             }
             memset(Gb, 0, sizeof(double) * (size_t)nnz);
             for (int s = 0; s < 4; ++s) {
@@ -616,7 +617,7 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
                     int e = off[s] + i;
                     if (!use[e]) continue;
                     double H = 0.0;
-                    for (int j = 0; j < m; ++j) H = H + A[(int64_t)j * m + i] * Phi[off[s] + j];
+                    for (int j = 0; j < m; ++j) H = fma(A[(int64_t)j * m + i], Phi[off[s] + j], H);
                     Gb[slot[e]] = Gb[slot[e]] + (-H);
                 }
             }

@@ -101,7 +101,7 @@ def math_eval(fn, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
     lib().orc_math_eval({"exp": 0, "log": 1, "sin": 2, "cos": 3, "tanh": 4, "sinpi": 5, "cospi": 6,
-                         "expm1": 7}[fn], ptr(x), ptr(y), x.size)
+                         "expm1": 7, "exp_tab": 8}[fn], ptr(x), ptr(y), x.size)
     return y
 
 

@@ -8,6 +8,8 @@
 
 namespace pdds {
 
+__constant__ double kExp2TabDev[64] = PDDS_EXP2_TAB;   // 2^(j/64), j = 0..63
+
 template <int PDE, int GEOM>
 static hipError_t path_f64_g(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
     if (!persistent)
@@ -88,6 +90,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
     __shared__ double s_wsum[(kReduceWG / 64) * kNBG];
+    __shared__ double s_exp2[64];
     __shared__ int s_lcnt[kNBG * 4];
     __shared__ unsigned long long s_steps;
     __shared__ unsigned int s_cnt[8];
@@ -118,6 +121,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         s_steps = 0ULL;
         for (int q = 0; q < 8; ++q) s_cnt[q] = 0u;
     }
+    if (tid < 64) s_exp2[tid] = kExp2TabDev[tid];
     __syncthreads();
     {
         unsigned long long my = 0ULL;
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
             } else if (lane == 1) {
                 // B sums of the sub-batch in path order
                 double bs = 0.0, bb = 0.0;
-                for (int p = p0; p < p1; ++p) { const double v = gs[p]; bs = bs + v; bb = bb + v * v; }
+                for (int p = p0; p < p1; ++p) { const double v = gs[p]; bs = bs + v; bb = __builtin_fma(v, v, bb); }
                 s_bsb[b] = bs;
                 s_bbsb[b] = bb;
             }
@@ -195,14 +199,14 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 int l = 0;
                 for (; l + 1 < nl; l += 2) {
                     const int pa = pb + L[l], pc = pb + L[l + 1];
-                    const double fa = gY[pa] * rbf(pxe, pye, gx[pa], gy[pa], ic2);
-                    const double fc = gY[pc] * rbf(pxe, pye, gx[pc], gy[pc], ic2);
-                    acc = acc + fa;
-                    acc = acc + fc;
+                    const double fa = rbf_tab(pxe, pye, gx[pa], gy[pa], ic2, s_exp2);
+                    const double fc = rbf_tab(pxe, pye, gx[pc], gy[pc], ic2, s_exp2);
+                    acc = __builtin_fma(gY[pa], fa, acc);
+                    acc = __builtin_fma(gY[pc], fc, acc);
                 }
                 if (l < nl) {
                     const int pa = pb + L[l];
-                    acc = acc + gY[pa] * rbf(pxe, pye, gx[pa], gy[pa], ic2);
+                    acc = __builtin_fma(gY[pa], rbf_tab(pxe, pye, gx[pa], gy[pa], ic2, s_exp2), acc);
                 }
                 Phi[e * kNBG + b] = acc;
             }
@@ -238,8 +242,8 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                         for (int b2 = 0; b2 < kNBG / 2; ++b2) {
                             const double2 v = ph[(j + u) * (kNBG / 2) + b2];
-                            H[2 * b2] = H[2 * b2] + av[u] * v.x;
-                            H[2 * b2 + 1] = H[2 * b2 + 1] + av[u] * v.y;
+                            H[2 * b2] = __builtin_fma(av[u], v.x, H[2 * b2]);
+                            H[2 * b2 + 1] = __builtin_fma(av[u], v.y, H[2 * b2 + 1]);
                         }
                     }
                 }
@@ -248,8 +252,8 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                     for (int b2 = 0; b2 < kNBG / 2; ++b2) {
                         const double2 v = ph[j * (kNBG / 2) + b2];
-                        H[2 * b2] = H[2 * b2] + aj * v.x;
-                        H[2 * b2 + 1] = H[2 * b2 + 1] + aj * v.y;
+                        H[2 * b2] = __builtin_fma(aj, v.x, H[2 * b2]);
+                        H[2 * b2 + 1] = __builtin_fma(aj, v.y, H[2 * b2 + 1]);
                     }
                 }
 #pragma unroll
@@ -267,7 +271,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
             if constexpr (EQ) {
                 const double wc = a.wref[kd.out_off + c];
 #pragma unroll
-                for (int b = 0; b < kNBG; ++b) tp[b] = tp[b] + G[b] * wc;
+                for (int b = 0; b < kNBG; ++b) tp[b] = __builtin_fma(G[b], wc, tp[b]);
             }
         }
         if constexpr (EQ) {

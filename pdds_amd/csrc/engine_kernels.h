@@ -77,10 +77,11 @@ PDDS_HD int classify_kind(int kind, int gmask) {
     return v;
 }
 
-// Gaussian RBF exp(-|p - X|^2 / c^2) (BVPs/BVP.hpp:84-87) with 1/c^2 precomputed
-PDDS_HD double rbf(double px, double py, double x, double y, double inv_c2) {
+// Gaussian RBF exp(-|p - X|^2 / c^2) (BVPs/BVP.hpp:84-87) with 1/c^2 precomputed; the table
+// driven exponential with its 2^(j/64) table T
+PDDS_HD double rbf_tab(double px, double py, double x, double y, double inv_c2, const double *T) {
     double dx = px - x, dy = py - y;
-    return dexp_nonpos(-(dx * dx + dy * dy) * inv_c2);
+    return dexp_tab_nonpos(-(dx * dx + dy * dy) * inv_c2, T);
 }
 
 // ----------------------------------------------------------------------------

@@ -43,6 +43,23 @@ def test_exp_log_expm1_tanh_ulps():
     assert abs(O.math_eval("exp", np.array([-744.0]))[0] - math.exp(-744.0)) <= 5e-324
 
 
+def test_table_exp_ulps_against_mpmath():
+    """The RBF reduction's table-driven exp (2^(j/64) table, degree-5 polynomial) on x <= 0:
+    within 2 ulp of the correctly rounded exp, exact at 0, subnormal results within one
+    subnormal unit, 0 below the underflow threshold.  A wrong table entry or coefficient fails."""
+    import mpmath as mp
+    mp.mp.prec = 120
+    rng = np.random.default_rng(4)
+    x = np.concatenate([-rng.exponential(2.0, 3000), rng.uniform(-700, 0, 3000), -np.arange(64) * math.log(2) / 64])
+    ref = np.array([float(mp.exp(mp.mpf(v))) for v in x])
+    got = O.math_eval("exp_tab", x)
+    assert ulp_err(got, ref).max() <= 2
+    assert O.math_eval("exp_tab", np.array([0.0]))[0] == 1.0
+    sub = np.array([-710.0, -730.5, -744.0])
+    assert np.all(np.abs(O.math_eval("exp_tab", sub) - np.array([math.exp(v) for v in sub])) <= 5e-324)
+    assert O.math_eval("exp_tab", np.array([-746.0]))[0] == 0.0
+
+
 def test_sinpi_cospi_and_sin_cos():
     rng = np.random.default_rng(2)
     x = rng.uniform(-50, 50, 200000)

@@ -140,8 +140,8 @@ __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uin
             float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
             float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
             float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
-            float r1 = __builtin_sqrtf(-2.0f * RM<float>::log(u1));
-            float r2 = __builtin_sqrtf(-2.0f * RM<float>::log(u3));
+            float r1 = RM<float>::sqrt(-2.0f * RM<float>::log(u1));
+            float r2 = RM<float>::sqrt(-2.0f * RM<float>::log(u3));
             n1 = r1 * __builtin_amdgcn_cosf(u2);
             n2 = r1 * __builtin_amdgcn_sinf(u2);
             s.pn1 = r2 * __builtin_amdgcn_cosf(u4);

@@ -293,10 +293,11 @@ template <> struct RM<double> {
     static PDDS_HD double fabs(double a) { return __builtin_fabs(a); }
 };
 template <> struct RM<float> {
-    // v_exp_f32 computes 2^x, v_log_f32 log2, v_sin/v_cos_f32 take revolutions
+    // v_exp_f32 computes 2^x, v_log_f32 log2, v_sin/v_cos_f32 take revolutions; v_sqrt_f32 is
+    // the hardware square root (1 ulp, no IEEE rounding fix-up: FP32 paths are not bit-exact)
     static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
     static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
-    static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ __forceinline__ void sincospi(float x, float &s, float &c) {
         float t = x * 0.5f;
         t = t - __builtin_floorf(t);
@@ -500,40 +501,25 @@ struct LutTile {
     int c0, r0, tw, th;
 };
 
-// Evaluate fields [0, NF) (NF <= 8) at (x,y):
-//   sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
-// With TILED, a lane whose 4x4 taps lie inside the tile reads fields 0..3 from LDS (one
-// 16-byte LDS read per tap) and the others read them from plane a; the values, and the
-// order of the fused multiply-adds, are the same either way.
-template <class R, int NF, bool TILED>
-PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF]) {
-    static_assert(NF <= 8, "at most 8 fields per texel");
-    R t = (x - (R)L.ox) * (R)L.sx;
-    R v = (y - (R)L.oy) * (R)L.sy;
-    t = RM<R>::fmin(RM<R>::fmax(t, (R)-2), (R)L.nx + (R)1);
-    v = RM<R>::fmin(RM<R>::fmax(v, (R)-2), (R)L.ny + (R)1);
-    const R fi = RM<R>::floor(t), fj = RM<R>::floor(v);
-    float wx[4], wy[4];
-    bspline_wf((float)(t - fi), wx);
-    bspline_wf((float)(v - fj), wy);
-    // padded column / row of tap (a=0, b=0): i - 1 + kLutPad >= 0, j - 1 + kLutPad >= 0
-    const int pc = (int)fi + (kLutPad - 1), pr = (int)fj + (kLutPad - 1);
-    const uint32_t t0 = (uint32_t)pr * (uint32_t)L.px + (uint32_t)pc;
-    bool in = false;
-    lds_float4 *tp = T.lds;
-    if constexpr (TILED) {
-        const int lc = pc - T.c0, lr = pr - T.r0;
-        in = lc >= 0 && lr >= 0 && lc + 3 < T.tw && lr + 3 < T.th;
-        tp = T.lds + (in ? lr * T.tw + lc : 0);
-    }
-    float acc[NF];
+// The 16-tap accumulation of lut_eval_t.  SRC 0: plane a from global memory; 1: from the LDS
+// tile; 2: per lane, the tile when `in`, else global memory.  The values and the order of the
+// fused multiply-adds are the same for every source.
+template <int NF, int SRC>
+PDDS_HD void lut_accum(const LutView &L, const LutTile &T, lds_float4 *tp, bool in, uint32_t t0, const float wx[4],
+                       const float wy[4], float acc[NF]) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0f;
+    // the fifth field's x-quads of the four rows first, so that their global loads are all in
+    // flight while the rows' plane-a taps are read
+    float4 Q[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        Q[b] = NF == 5 ? lut_at<float4>(L.b, (t0 + (uint32_t)b * (uint32_t)L.px) * 16u) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t r0 = t0 + (uint32_t)b * (uint32_t)L.px;
         float4 A[4];
-        if (TILED && in) {
+        if (SRC == 1 || (SRC == 2 && in)) {
 #pragma unroll
             for (int a = 0; a < 4; ++a) A[a] = tp[b * T.tw + a];
         } else {
@@ -543,8 +529,7 @@ PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF])
         float row[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) row[f] = 0.0f;
-        float4 quad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if constexpr (NF == 5) quad = lut_at<float4>(L.b, r0 * 16u);
+        const float4 quad = Q[b];
         const float f4[4] = {quad.x, quad.y, quad.z, quad.w};
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -562,6 +547,43 @@ PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF])
         }
 #pragma unroll
         for (int f = 0; f < NF; ++f) acc[f] = __builtin_fmaf(wy[b], row[f], acc[f]);
+    }
+}
+
+// Evaluate fields [0, NF) (NF <= 8) at (x,y):
+//   sum_b wy_b * sum_a wx_a * C[clamp(j-1+b)][clamp(i-1+a)][f]
+// With TILED, a lane whose 4x4 taps lie inside the tile reads fields 0..3 from LDS (one
+// 16-byte LDS read per tap) and the others read them from plane a; the values, and the
+// order of the fused multiply-adds, are the same either way.  A wave whose lanes are all
+// inside the tile takes a path with LDS reads only, so no LDS read waits for the wave's
+// global loads.
+template <class R, int NF, bool TILED>
+PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF]) {
+    static_assert(NF <= 8, "at most 8 fields per texel");
+    R t = (x - (R)L.ox) * (R)L.sx;
+    R v = (y - (R)L.oy) * (R)L.sy;
+    t = RM<R>::fmin(RM<R>::fmax(t, (R)-2), (R)L.nx + (R)1);
+    v = RM<R>::fmin(RM<R>::fmax(v, (R)-2), (R)L.ny + (R)1);
+    const R fi = RM<R>::floor(t), fj = RM<R>::floor(v);
+    float wx[4], wy[4];
+    bspline_wf((float)(t - fi), wx);
+    bspline_wf((float)(v - fj), wy);
+    // padded column / row of tap (a=0, b=0): i - 1 + kLutPad >= 0, j - 1 + kLutPad >= 0
+    const int pc = (int)fi + (kLutPad - 1), pr = (int)fj + (kLutPad - 1);
+    const uint32_t t0 = (uint32_t)pr * (uint32_t)L.px + (uint32_t)pc;
+    float acc[NF];
+    if constexpr (TILED) {
+        const int lc = pc - T.c0, lr = pr - T.r0;
+        const bool in = lc >= 0 && lr >= 0 && lc + 3 < T.tw && lr + 3 < T.th;
+        lds_float4 *tp = T.lds + (in ? lr * T.tw + lc : 0);
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (__all(in)) lut_accum<NF, 1>(L, T, tp, in, t0, wx, wy, acc);
+        else lut_accum<NF, 2>(L, T, tp, in, t0, wx, wy, acc);
+#else
+        lut_accum<NF, 2>(L, T, tp, in, t0, wx, wy, acc);
+#endif
+    } else {
+        lut_accum<NF, 0>(L, T, T.lds, false, t0, wx, wy, acc);
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) out[f] = (R)acc[f];

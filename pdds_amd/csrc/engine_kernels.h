@@ -504,7 +504,9 @@ path_kernel_tiled(PathKernelArgs a) {
                 if (can_start) break;
                 continue;
             }
-            w.template step<true, true>(a, T, it);
+            // FP64 overlaps the next step's normal draw with the LUT reads; the shorter FP32
+            // draw goes first (measured: each order is about 1 % faster in its precision)
+            w.template step<true, sizeof(R) == 4>(a, T, it);
         }
         __syncthreads();   // every wave is done with this tile
     }

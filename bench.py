@@ -348,6 +348,18 @@ def main():
                             "kernel_ms": o2["counters"]["kernel_ms"], "rms_interface_error": e2}
         extras["engine_fp64"] = {"path_steps_per_s": value, "seconds": wall / args.steps, "rms_interface_error": rms}
         extras["engine_fp32_vs_plain_fp64"] = extras["engine_fp32"]["path_steps_per_s"] / extras["plain_fp64"]["path_steps_per_s"]
+        # time to an interface solution at least as accurate as the plain kernel's: the batching form
+        # (FP32 paths, variance-targeted counts) against one plain FP64 assembly plus solve
+        ts = time.perf_counter()
+        ad = eng.solve_adaptive(mesh, prob, default_run(h=h, seed=seed, precision="fp32"), se_target=1e-3,
+                                n_min=1000, n_max=1000000, max_rounds=4)
+        dt = time.perf_counter() - ts
+        ad_rms = float(np.sqrt(np.mean((ad["u"] - prob.u(mesh.px, mesh.py)) ** 2)))
+        extras["adaptive_fp32"] = {"seconds": dt, "rms_interface_error": ad_rms, "paths": ad["report"]["paths"],
+                                   "path_steps": ad["report"]["path_steps"], "rounds": ad["report"]["rounds"],
+                                   "max_se": ad["report"]["max_se"]}
+        if ad_rms <= extras["plain_fp64"]["rms_interface_error"]:
+            extras["time_to_rms_plain_fp64_over_adaptive_fp32"] = extras["plain_fp64"]["seconds"] / dt
         result["forms"] = extras
     if not args.no_serving and not args.no_extras:
         result["serving"] = serving_bench(eng, args.serving_seconds)

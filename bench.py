@@ -203,17 +203,28 @@ def batching_bench(eng):
     rep = out["report"]
     mc = ~mesh.on_boundary
     N = out["paths_per_knot"][mc]
-    err = out["u"] - prob.u(mesh.px, mesh.py)
+    uex = prob.u(mesh.px, mesh.py)
+    err = out["u"] - uex
+    # the same number of paths spread evenly over the knots, one assembly and solve
+    n_even = max(1, int(round(rep["paths"] / mc.sum())))
+    t2 = time.perf_counter()
+    ev = eng.assemble(mesh, prob, best, n_even)
+    u_even = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, ev["G"], ev["B"], tol=1e-10)[0]
+    t_even = time.perf_counter() - t2
     return {
         "workload": "disc r=1, 6 rings x 24 sectors, 8..64 knots/interface (seed 11): %d knots (%d MC), "
-                    "cfg2 PDE from 512^2 LUTs, h=%g, FP64 paths, SE target %g per knot equation, N in [%d, %d]"
+                    "cfg2 PDE from 512^2 LUTs, h=%g, FP64 paths, SE target %g per knot score (the reference's "
+                    "variance rule), N in [%d, %d]"
                     % (mesh.n_knots, int(mc.sum()), h, cfg["se_target"], cfg["n_min"], cfg["n_max"]),
         "path_steps_per_s": rep["path_steps"] / wall, "seconds": wall, "mesh_build_s": t_mesh,
         "rounds": rep["rounds"], "paths": rep["paths"], "path_steps": rep["path_steps"],
         "paths_per_knot": {"min": int(N.min()), "median": int(np.median(N)), "p99": int(np.percentile(N, 99)),
                            "max": int(N.max())},
-        "max_se": rep["max_se"], "knots_capped": rep["knots_capped"], "kernel_ms": rep["kernel_ms"],
+        "max_se": rep["max_se"], "knots_over": rep["knots_over"], "knots_capped": rep["knots_capped"],
+        "kernel_ms": rep["kernel_ms"],
         "rms_interface_error": float(np.sqrt(np.mean(err ** 2))),
+        "even_counts": {"paths_per_knot": n_even, "path_steps": ev["counters"]["path_steps"], "seconds": t_even,
+                        "rms_interface_error": float(np.sqrt(np.mean((u_even - uex) ** 2)))},
         "autotune": {"best": {k: getattr(best, k) for k in ("lut_lds", "wg_size", "grid_wgs", "paths_per_lane")},
                      "candidates": tune["candidates"], "best_rate": tune["best_rate"], "base_rate": tune["base_rate"],
                      "ms": tune["total_ms"]},
@@ -330,8 +341,8 @@ def main():
                                 if c["wg_size"] != 256 else "path_kernel (persistent, LUT from global memory)"),
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
                      "note": ("no dense contraction: peak = MI355X %s; achieved counts %g algorithmic flops per "
-                              "path-step; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE (copy-calibrated), "
-                              "mostly scattered exit-record stores"
+                              "path-step; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE (copy-calibrated): "
+                              "exit records stored when each path stops, and fifth-field LUT quads missing L2"
                               % ("FP64 vector (= FP64 matrix) 78.6 TFLOP/s" if args.precision == "fp64"
                                  else "FP32 vector 157.3 TFLOP/s (packed; this build issues unpacked FP32)",
                                  FLOPS_PER_STEP.get(args.config, 302.0)))},
@@ -348,18 +359,21 @@ def main():
                             "kernel_ms": o2["counters"]["kernel_ms"], "rms_interface_error": e2}
         extras["engine_fp64"] = {"path_steps_per_s": value, "seconds": wall / args.steps, "rms_interface_error": rms}
         extras["engine_fp32_vs_plain_fp64"] = extras["engine_fp32"]["path_steps_per_s"] / extras["plain_fp64"]["path_steps_per_s"]
-        # time to an interface solution at least as accurate as the plain kernel's: the batching form
-        # (FP32 paths, variance-targeted counts) against one plain FP64 assembly plus solve
-        ts = time.perf_counter()
-        ad = eng.solve_adaptive(mesh, prob, default_run(h=h, seed=seed, precision="fp32"), se_target=1e-3,
-                                n_min=1000, n_max=1000000, max_rounds=4)
-        dt = time.perf_counter() - ts
-        ad_rms = float(np.sqrt(np.mean((ad["u"] - prob.u(mesh.px, mesh.py)) ** 2)))
-        extras["adaptive_fp32"] = {"seconds": dt, "rms_interface_error": ad_rms, "paths": ad["report"]["paths"],
-                                   "path_steps": ad["report"]["path_steps"], "rounds": ad["report"]["rounds"],
-                                   "max_se": ad["report"]["max_se"]}
-        if ad_rms <= extras["plain_fp64"]["rms_interface_error"]:
-            extras["time_to_rms_plain_fp64_over_adaptive_fp32"] = extras["plain_fp64"]["seconds"] / dt
+        # the batching form (variance-targeted counts, pilot and rounds included) on the FP32 engine
+        # and on the plain FP64 kernel; on this workload the RMS error is set by the time step and
+        # the RBF interpolation (it is the same from 1e3 to 1e5 paths per knot), not by the counts
+        for name, kw in (("adaptive_fp32", dict(precision="fp32")), ("adaptive_plain_fp64", dict(plain=1))):
+            ts = time.perf_counter()
+            ad = eng.solve_adaptive(mesh, prob, default_run(h=h, seed=seed, **kw), se_target=4e-3,
+                                    n_min=1000, n_max=1000000, max_rounds=4)
+            dt = time.perf_counter() - ts
+            ad_rms = float(np.sqrt(np.mean((ad["u"] - prob.u(mesh.px, mesh.py)) ** 2)))
+            extras[name] = {"seconds": dt, "rms_interface_error": ad_rms, "paths": ad["report"]["paths"],
+                            "path_steps": ad["report"]["path_steps"], "rounds": ad["report"]["rounds"],
+                            "max_se": ad["report"]["max_se"], "knots_over": ad["report"]["knots_over"],
+                            "kernel_ms": ad["report"]["kernel_ms"]}
+        extras["adaptive_plain_fp64_over_adaptive_fp32"] = (extras["adaptive_plain_fp64"]["seconds"]
+                                                            / extras["adaptive_fp32"]["seconds"])
         result["forms"] = extras
     if not args.no_serving and not args.no_extras:
         result["serving"] = serving_bench(eng, args.serving_seconds)

@@ -192,11 +192,26 @@ pdds_status pdds_assemble_eq(pdds_engine *e, const pdds_mesh *m, const pdds_prob
 
 /* ---- variance-targeted path counts (batching form; the reference's policy is commented out,
    PDDAlgorithms/PDDSparseGM.cpp:4358-4365) ----
-   A pilot assembly and solve, then rounds that raise each knot's path count to
-   growth * eq_var_k / se_target^2 (within [n_min, n_max]) until every knot's equation standard
-   error is at most se_target or its count is n_max.  Every round is one assembly of all knots with
-   their uneven counts in one persistent launch, followed by the device solve.  Outputs: u (n_knots),
-   the final counts (0 for boundary knots) and, if se is not NULL, each knot's standard error. */
+   A pilot assembly, then rounds that move each knot's path count to growth * var_k / se_target^2
+   (within [n_min, n_max]) until every knot's standard error sqrt(var_k / N_k) is at most se_target
+   or its count is n_max.  var_k is the batch-means variance of
+     variance = 0 (default): the knot's score, control variate included -- the estimator variance
+                 Var(phi) (1 - rho^2) of the reference's rule (PDDSparseGM.cpp:4358-4359);
+     variance = 1: the knot's equation B_k - sum_j G_kj u_ref_j, u_ref the path-weighted mean of
+                 the rounds' solutions (a pilot solve first).  Far smaller than the score's where
+                 u_ref is accurate, but the stencil sides' RBF cardinal functions amplify the
+                 knot-to-knot noise of a u_ref solved from few paths per knot, so the measured
+                 variance can exceed the true one many times over (seen on disc meshes).  Every round is one assembly of all knots with
+   their uneven counts in one persistent launch, followed by the device solve.  Round a (0 = the
+   pilot) draws its paths from Philox stream r->stream + a * 0x9E3779B9 (mod 2^32), so the counts
+   of a round, set from earlier rounds' paths, never depend on its own paths (re-using the pilot's
+   paths would bias the estimate: a knot whose first paths happen to vary more gets more paths
+   added to them).  Outputs: u (n_knots), the final counts (0 for boundary knots) and, if se is not
+   NULL, each knot's standard error; u is the solve of one pdds_assemble with those counts and
+   rep->stream.  A knot's standard error is measured on the final round's own paths, so it
+   scatters around the value its count was set for by the sampling error of the batch-means
+   variance (about sqrt(2 / batches)); growth is the margin against it, and rep->knots_over counts
+   the knots that end above se_target. */
 typedef struct {
     double se_target;       /* target standard error of every knot equation (default 1e-3) */
     int64_t n_min, n_max;   /* per-knot path-count bounds (defaults 1e3, 1e6) */
@@ -204,6 +219,7 @@ typedef struct {
     int32_t max_rounds;     /* refinement rounds after the pilot (default 3) */
     double tol;             /* BiCGSTAB relative tolerance (default 1e-10) */
     double growth;          /* safety factor on the variance-targeted count (default 1.1) */
+    int32_t variance;       /* 0: score variance (default), 1: equation variance */
 } pdds_adaptive;
 typedef struct {
     int32_t rounds;         /* refinement rounds run */
@@ -215,6 +231,8 @@ typedef struct {
     int32_t knots_capped;   /* knots held at n_max above the target */
     double kernel_ms;       /* device time of the path and reduction kernels, all assemblies */
     double total_ms;        /* host wall time */
+    uint32_t stream;        /* Philox stream of the final assembly */
+    int32_t knots_over;     /* knots whose final standard error exceeds se_target */
 } pdds_adaptive_report;
 void pdds_adaptive_defaults(pdds_adaptive *c);
 pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,

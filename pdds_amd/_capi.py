@@ -34,13 +34,14 @@ class Run(C.Structure):
 
 class Adaptive(C.Structure):
     _fields_ = [("se_target", C.c_double), ("n_min", C.c_int64), ("n_max", C.c_int64), ("n_pilot", C.c_int64),
-                ("max_rounds", C.c_int32), ("tol", C.c_double), ("growth", C.c_double)]
+                ("max_rounds", C.c_int32), ("tol", C.c_double), ("growth", C.c_double), ("variance", C.c_int32)]
 
 
 class AdaptiveReport(C.Structure):
     _fields_ = [("rounds", C.c_int32), ("iters", C.c_int32), ("relres", C.c_double), ("paths", C.c_int64),
                 ("path_steps", C.c_int64), ("max_se", C.c_double), ("knots_capped", C.c_int32),
-                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("stream", C.c_uint32),
+                ("knots_over", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

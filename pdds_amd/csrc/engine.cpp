@@ -153,7 +153,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     const int64_t max_steps = r->max_steps > 0 ? r->max_steps : (int64_t)1 << 26;
     if (r->item_paths > 0 && r->item_paths % S != 0)
         return fail(e, PDDS_ERR_INVALID, "item_paths must be a multiple of sub_batch");
-    if (max_steps >= kCappedBit) return fail(e, PDDS_ERR_INVALID, "max_steps must be below 2^30");
+    if (max_steps > kStepMask) return fail(e, PDDS_ERR_INVALID, "max_steps must be below 2^27");
     out.S = S;
     out.wg = wg;
     out.kstats.assign(knots.size(), KnotStat{});
@@ -185,7 +185,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         out.wg = path_tiled_wg(r->precision);
     }
     // knots are processed in batches whose exit records fit the record budget
-    const int64_t rec_bytes = 4 * 8 + 1 + 4;
+    const int64_t rec_bytes = (int64_t)sizeof(ExitRec) + 4;
     const int64_t budget = e->record_budget / rec_bytes;
     void *p;
     double *dG, *dGv;
@@ -253,12 +253,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         if ((st = upload(e, B_KNOTS, kd, &dk))) return st;
         if ((st = upload(e, B_ITEMS, items, &dit))) return st;
         Records rec;
-        if ((st = dbuf(e, B_RECX, (size_t)tot * 8, &p))) return st; rec.x = (double *)p;
-        if ((st = dbuf(e, B_RECY, (size_t)tot * 8, &p))) return st; rec.y = (double *)p;
-        if ((st = dbuf(e, B_RECYY, (size_t)tot * 8, &p))) return st; rec.Y = (double *)p;
-        if ((st = dbuf(e, B_RECS, (size_t)tot * 8, &p))) return st; rec.score = (double *)p;
-        if ((st = dbuf(e, B_RECSIDE, (size_t)tot, &p))) return st; rec.side = (int8_t *)p;
-        if ((st = dbuf(e, B_RECSTEPS, (size_t)tot * 4, &p))) return st; rec.steps = (int32_t *)p;
+        if ((st = dbuf(e, B_REC, (size_t)tot * sizeof(ExitRec), &p))) return st; rec.r = (ExitRec *)p;
+        if ((st = dbuf(e, B_RECMETA, (size_t)tot * 4, &p))) return st; rec.meta = (int32_t *)p;
         ItemStat *dstats; double *dpart; KnotStat *dks; unsigned long long *dctr;
         if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st; dstats = (ItemStat *)p;
         if ((st = dbuf(e, B_PART, (size_t)std::max<int64_t>(part, 1) * sizeof(double), &p))) return st; dpart = (double *)p;
@@ -861,28 +857,31 @@ void pdds_adaptive_defaults(pdds_adaptive *c) {
     c->max_rounds = 3;
     c->tol = 1e-10;
     c->growth = 1.1;
+    c->variance = 0;
 }
 
-// Pilot assembly and solve, then rounds of: assemble with the current counts, solve, and move
-// each knot's count towards N_k = growth * var_k / se^2 (within [n_min, n_max]), var_k the
-// batch-means variance of its equation sqrt(var_k / N_k) being its standard error.  Every round
-// re-runs paths [0, N_k) of each knot, so the final G, B and u are those of one plain assembly
-// with the returned counts.  The policy stands in for the
-// commented Compute_h_N of the reference (PDDAlgorithms/PDDSparseGM.cpp:4338-4395, N_i =
-// 4 Var_i (1 - rho_i^2) / (0.6667 eps)^2) with the batch-means variance of the knot equation,
-// control variate included, in place of its Var_i (1 - rho_i^2).
+// Pilot assembly, then rounds of: assemble with the current counts, solve, and move each
+// knot's count towards N_k = growth * var_k / se^2 (within [n_min, n_max]), var_k the
+// batch-means variance of its score (variance = 0) or of its equation weighed with a reference
+// solution (variance = 1), sqrt(var_k / N_k) being its standard error.  Each round draws fresh
+// paths (a Philox stream of its own), so a round's counts do not depend on its own paths, and
+// the final G, B and u are those of one plain assembly with the returned counts and stream.
+// The policy stands in for the commented Compute_h_N of the reference
+// (PDDAlgorithms/PDDSparseGM.cpp:4338-4395, N_i = 4 Var_i (1 - rho_i^2) / (0.6667 eps)^2, the
+// variance of the control-variate estimator with a standard error of eps/3): variance = 0 is the
+// batch-means variance of our score, whose control variate is built in.
 pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
                                 const pdds_adaptive *cfg_in, double *u, int64_t *paths_per_knot, double *se,
                                 pdds_adaptive_report *rep) {
     if (!e) return PDDS_ERR_INVALID;
     std::lock_guard<std::mutex> lk(e->mu);
     const double t0 = now_ms();
-    if (!m || !u || !paths_per_knot) return fail(e, PDDS_ERR_INVALID, "null mesh or output");
+    if (!m || !r || !u || !paths_per_knot) return fail(e, PDDS_ERR_INVALID, "null mesh, run or output");
     pdds_adaptive cfg;
     pdds_adaptive_defaults(&cfg);
     if (cfg_in) cfg = *cfg_in;
     if (!(cfg.se_target > 0) || cfg.n_min < 1 || cfg.n_max < cfg.n_min || cfg.max_rounds < 1 || cfg.max_rounds > 64 ||
-        !(cfg.growth >= 1.0))
+        !(cfg.growth >= 1.0) || cfg.variance < 0 || cfg.variance > 1)
         return fail(e, PDDS_ERR_INVALID, "bad adaptive configuration");
     const int64_t n_pilot = cfg.n_pilot > 0 ? cfg.n_pilot : cfg.n_min;
     const Mesh &M = m->m;
@@ -896,9 +895,12 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
     const double tol = cfg.tol > 0 ? cfg.tol : 1e-10;
     int32_t it = 0;
     double rr = 0.0;
+    // every round draws fresh paths: round a uses stream r->stream + a * kRoundStream
+    constexpr uint32_t kRoundStream = 0x9E3779B9u;
+    pdds_run run = *r;
     auto assemble = [&](const double *ur) -> pdds_status {
         AsmOut o;
-        pdds_status st = assemble_locked(e, m, p, r, N.data(), 0, ur, G.data(), Gv.data(), B.data(), BB.data(),
+        pdds_status st = assemble_locked(e, m, p, &run, N.data(), 0, ur, G.data(), Gv.data(), B.data(), BB.data(),
                                          em.data(), ev.data(), o);
         if (st) return st;
         for (const KnotStat &k : o.kstats) R.path_steps += (int64_t)k.steps;
@@ -908,16 +910,32 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
     auto solve = [&](double *out) -> pdds_status {
         return solve_system(e, n, M.row_ptr.data(), M.cols.data(), G.data(), B.data(), out, tol, 0, &it, &rr);
     };
-    // The equations are weighed with the most accurate solution so far (the one from the
-    // assembly with the most paths): a noisy reference inflates the measured variance through
-    // the interpolation of its noise along the stencil sides.  Counts move towards
-    // growth * var / se^2 by at most a factor 8 per round, up or down, so that an over-estimate
-    // from an early, noisy reference is taken back once the reference improves.
+    // The equations are weighed with a reference solution.  A noisy reference inflates the
+    // measured variance: the RBF cardinal functions of the stencil sides amplify its knot-to-knot
+    // noise at the exit points.  Every round draws fresh paths, so the rounds' solutions are
+    // independent estimates of u, and the reference is their mean weighted by each round's paths,
+    // whose noise falls as rounds accumulate.  Counts move towards growth * var / se^2 by at most
+    // a factor 8 per round, up or down, so that an over-estimate from an early, noisy reference is
+    // taken back once the reference improves.
+    // With variance = 0 the weights are zero, so that the equation statistics are the score's.
+    const bool eq_var = cfg.variance == 1;
     pdds_status st;
-    if ((st = assemble(nullptr)) || (st = solve(uref.data())) || (st = assemble(uref.data()))) return st;
-    int64_t best_paths = 0;
-    for (int q = 0; q < n; ++q)
-        if (!M.on_boundary[q]) best_paths += N[q];
+    if (eq_var) {
+        if ((st = assemble(nullptr)) || (st = solve(uref.data())) || (st = assemble(uref.data()))) return st;
+    } else {
+        std::fill(uref.begin(), uref.end(), 0.0);
+        if ((st = assemble(uref.data()))) return st;
+    }
+    std::vector<double> usum(n, 0.0);
+    double wsum = 0.0;
+    auto add_solution = [&](const double *v) {
+        double w = 0.0;
+        for (int q = 0; q < n; ++q)
+            if (!M.on_boundary[q]) w += (double)N[q];
+        for (int q = 0; q < n; ++q) usum[q] += w * v[q];
+        wsum += w;
+    };
+    if (eq_var) add_solution(uref.data());
     auto target = [&](int q) -> int64_t {
         const double want = std::ceil(cfg.growth * ev[q] / (cfg.se_target * cfg.se_target));
         double c = std::min(std::max(want, (double)cfg.n_min), (double)cfg.n_max);
@@ -928,7 +946,9 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
     for (int q = 0; q < n; ++q)
         if (!M.on_boundary[q]) N[q] = target(q);
     for (int round = 0; round < cfg.max_rounds; ++round) {
+        run.stream = r->stream + (uint32_t)(round + 1) * kRoundStream;
         if ((st = assemble(uref.data())) || (st = solve(u))) return st;
+        if (eq_var) add_solution(u);
         R.rounds = round + 1;
         bool over = false, moved = false;
         std::vector<int64_t> next(N);
@@ -944,18 +964,14 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
         }
         // the counts returned are those of the last assembly
         if ((!over && !moved) || round + 1 == cfg.max_rounds) break;
-        int64_t tot = 0;
-        for (int q = 0; q < n; ++q)
-            if (!M.on_boundary[q]) tot += N[q];
-        if (tot >= best_paths) {
-            best_paths = tot;
-            std::copy(u, u + n, uref.begin());
-        }
+        if (eq_var && wsum > 0.0)
+            for (int q = 0; q < n; ++q) uref[q] = usum[q] / wsum;
         N.swap(next);
     }
     R.paths = 0;
     R.max_se = 0.0;
     R.knots_capped = 0;
+    R.knots_over = 0;
     for (int q = 0; q < n; ++q) {
         paths_per_knot[q] = M.on_boundary[q] ? 0 : N[q];
         const double sek = M.on_boundary[q] ? 0.0 : std::sqrt(std::max(ev[q], 0.0) / (double)N[q]);
@@ -963,10 +979,12 @@ pdds_status pdds_solve_adaptive(pdds_engine *e, const pdds_mesh *m, const pdds_p
         if (M.on_boundary[q]) continue;
         R.paths += N[q];
         R.max_se = std::max(R.max_se, sek);
+        if (sek > cfg.se_target) R.knots_over++;
         if (sek > cfg.se_target && N[q] >= cfg.n_max) R.knots_capped++;
     }
     R.iters = it;
     R.relres = rr;
+    R.stream = run.stream;
     R.total_ms = now_ms() - t0;
     if (rep) *rep = R;
     return PDDS_OK;

@@ -103,9 +103,8 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     const TableDesc tb = a.tables[kd.table];
     const StencilDesc &sd = tb.sten[kd.stencil];
     const int64_t g0 = kd.path_off + it.p_begin;
-    const double *rx = a.rec.x + g0, *ry = a.rec.y + g0, *rY = a.rec.Y + g0, *rs = a.rec.score + g0;
-    const int8_t *rside = a.rec.side + g0;
-    const int32_t *rst = a.rec.steps + g0;
+    const ExitRec *rr = a.rec.r + g0;
+    const int32_t *rmeta = a.rec.meta + g0;
     const int S = a.S;
     const int GS = kNBG * S;
     double *Phi = smem;                                     // [ne_cap][kNBG]
@@ -127,10 +126,10 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         unsigned long long my = 0ULL;
         unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int p = tid; p < it.n; p += WG) {
-            const int32_t st = rst[p];
-            my += (unsigned long long)(st & ~kCappedBit);
-            if (st & kCappedBit) cnt[6]++;
-            const int sd0 = rside[p];
+            const int32_t mt = rmeta[p];
+            my += (unsigned long long)(mt & kStepMask);
+            if (mt & kCappedBit) cnt[6]++;
+            const int sd0 = (mt >> kSideShift) - 1;
             cnt[sd0 < 0 ? 5 : sd0]++;
         }
         atomicAdd(&s_steps, my);
@@ -150,11 +149,12 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         const int q0 = g * S;                                   // first path of the group
         const int gn = (it.n - q0) < nb * S ? (it.n - q0) : nb * S;
         for (int p = tid; p < gn; p += WG) {
-            gx[p] = rx[q0 + p];
-            gy[p] = ry[q0 + p];
-            gY[p] = rY[q0 + p];
-            gs[p] = rs[q0 + p];
-            gside[p] = rside[q0 + p];
+            const ExitRec rc = rr[q0 + p];
+            gx[p] = rc.x;
+            gy[p] = rc.y;
+            gY[p] = rc.Y;
+            gs[p] = rc.score;
+            gside[p] = (int8_t)((rmeta[q0 + p] >> kSideShift) - 1);
         }
         __syncthreads();
         // per (sub-batch, side) lists in path order: a wave per sub-batch, 64 paths per ballot

@@ -197,12 +197,13 @@ __device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, 
     } else {
         sc = (double)(s.Z + s.xi);
     }
-    a.rec.x[g] = (double)ex;
-    a.rec.y[g] = (double)ey;
-    a.rec.Y[g] = (double)s.Y;
-    a.rec.score[g] = sc;
-    a.rec.side[g] = (int8_t)side;
-    a.rec.steps[g] = (int32_t)s.st | (capped_inside ? kCappedBit : 0);
+    ExitRec rc;
+    rc.x = (double)ex;
+    rc.y = (double)ey;
+    rc.Y = (double)s.Y;
+    rc.score = sc;
+    a.rec.r[g] = rc;
+    a.rec.meta[g] = (int32_t)s.st | (capped_inside ? kCappedBit : 0) | ((side + 1) << kSideShift);
 }
 
 // Distance to a stencil's stopping domain: its rectangle, or its annular sector on a disc mesh.

@@ -17,7 +17,12 @@ inline size_t reduce_lds_bytes(int ne_cap, int nnz_cap, int S) {
     b += g * 4 * sizeof(uint16_t) + g;
     return (b + 15) & ~(size_t)15;
 }
-constexpr int32_t kCappedBit = 1 << 30;    // record flag: path stopped by the step cap
+// Exit-record metadata word: the step count in bits 0..26, bit 27 set when the path was
+// stopped by the step cap, the exit side + 1 (0 = none, 1..6 = sides -1..5) in bits 28..30.
+constexpr int32_t kStepBits = 27;
+constexpr int32_t kStepMask = (1 << kStepBits) - 1;
+constexpr int32_t kCappedBit = 1 << kStepBits;
+constexpr int32_t kSideShift = 28;
 constexpr int kMaxQueues = 8;              // path queues: one per XCD of an MI355X (8 XCDs x 32 CUs)
 
 // One Monte Carlo knot of an assembly batch.
@@ -85,11 +90,15 @@ struct KnotStat {
     unsigned long long cnt[6];
 };
 
-// Per-path exit records, indexed by global path index.
+// Per-path exit records, indexed by global path index: the FP64 exit values as one aligned
+// 32-byte record (a lane stores it with two 16-byte writes when its path stops, so a record
+// reaches memory as one 32-byte piece), and the metadata word (steps, cap flag, side).
+struct alignas(32) ExitRec {
+    double x, y, Y, score;
+};
 struct Records {
-    double *x, *y, *Y, *score;
-    int8_t *side;
-    int32_t *steps;
+    ExitRec *r;
+    int32_t *meta;
 };
 
 // A patch: knots whose LUT tile (the plane-a texels their stencil rectangle can touch) is the

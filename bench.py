@@ -65,9 +65,31 @@ def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
         steps += int(paths["steps"].sum())
         used.append(k)
     t_sim = time.perf_counter() - t
+    # the per-core rate: one thread on the first sampled knot
+    run1 = O.run_params(h, seed, threads=1)
+    k = int(order[0])
+    st = mesh.stencil(k)
+    t1 = time.perf_counter()
+    p1 = O.simulate(pb, run1, mesh.x0[k], mesh.y0[k], st["rect"], k, 0, 4096)
+    rate1 = int(p1["steps"].sum()) / (time.perf_counter() - t1)
+    # the reduction of one sampled knot's paths, timed apart from the stepping (one thread): the
+    # engine-order batched form on all of them, the reference's per-exit form (m RBF values and an
+    # m x m product with Psi^-T per exit) on 2048
+    paths = O.simulate(pb, run, mesh.x0[k], mesh.y0[k], st["rect"], k, 0, chunk)
+    cs = st["cshape"]
+    t2 = time.perf_counter()
+    O.reduce_batched(pb, paths, st, mesh.glob, cs, st["ipsi"], 32, 6272)
+    t_red = (time.perf_counter() - t2) / chunk
+    sub = {q: v[:2048] for q, v in paths.items()}
+    t3 = time.perf_counter()
+    O.reduce_faithful(pb, sub, st, mesh.glob, cs, st["ipsi"])
+    t_faith = (time.perf_counter() - t3) / 2048
     return {"value": steps / t_sim, "unit": "path-steps/s", "cores": threads, "kind": "port",
             "sample": "%d knots x %d paths (seeded random interior knots of %d), %d path-steps, phase-1 "
                       "stepping, FP64 C/OpenMP oracle, %.1f s" % (len(used), chunk, len(interior), steps, t_sim),
+            "one_thread_path_steps_per_s": rate1,
+            "reduction_s_per_path_one_thread": {"knot": k, "stencil_entries": int(np.sum(st["msz"])),
+                                                "batched_engine_order": t_red, "per_exit_reference_form": t_faith},
             "cpu": platform.processor() or platform.machine()}
 
 

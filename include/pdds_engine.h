@@ -173,7 +173,11 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
                             pdds_counters *ctr);
 
 /* ---- every knot of a mesh: G values in the mesh CSR pattern, per-entry per-path variance,
-   B and B second moment per knot.  paths_per_knot may be NULL (n_paths each). ---- */
+   B and B second moment per knot.  paths_per_knot may be NULL (n_paths each, n_paths >= 1).
+   A count of 0 for a Monte Carlo knot leaves it out of this call (its row, variances, B and BB
+   are written as 0): one shard of a split by knots.  A knot's results do not depend on which
+   other knots share the call, so the shards' rows put together equal one call over all knots.
+   ---- */
 pdds_status pdds_assemble(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
                           const int64_t *paths_per_knot, int64_t n_paths, double *G, double *G_var,
                           double *B, double *BB, pdds_counters *ctr);

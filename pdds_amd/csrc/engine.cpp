@@ -786,7 +786,7 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
     DevTables *T;
     if ((st = mesh_dev_tables(e, m, r->quirk_last_side, &T))) return st;
     std::vector<AsmKnot> knots;
-    std::vector<int> kmap;
+    std::vector<int> kmap, skipped;
     for (int q = 0; q < M.n_knots(); ++q) {
         int64_t n = paths_per_knot ? paths_per_knot[q] : n_paths;
         if (n < 0) return fail(e, PDDS_ERR_INVALID, "negative path count");
@@ -797,7 +797,12 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
             if (eq_var) eq_var[q] = 0.0;
             continue;
         }
-        if (n == 0) return fail(e, PDDS_ERR_INVALID, "interior knot with zero paths");
+        if (n == 0) {
+            // with per-knot counts, 0 leaves the knot to another shard of a split by knots
+            if (!paths_per_knot) return fail(e, PDDS_ERR_INVALID, "n_paths must be positive");
+            skipped.push_back(q);
+            continue;
+        }
         AsmKnot k{M.x0[q], M.y0[q], (uint32_t)q, M.stencil_of[q], n, M.row_ptr[q]};
         knots.push_back(k);
         kmap.push_back(q);
@@ -814,6 +819,16 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
         BB[kmap[i]] = o.kstats[i].BB;
         if (eq_mean) eq_mean[kmap[i]] = u_ref ? o.kstats[i].Rm : 0.0;
         if (eq_var) eq_var[kmap[i]] = u_ref ? o.kstats[i].Rv : 0.0;
+    }
+    for (int q : skipped) {
+        for (int64_t i = M.row_ptr[q]; i < M.row_ptr[q + 1]; ++i) {
+            if (G) G[i] = 0.0;
+            if (G_var) G_var[i] = 0.0;
+        }
+        B[q] = 0.0;
+        BB[q] = 0.0;
+        if (eq_mean) eq_mean[q] = 0.0;
+        if (eq_var) eq_var[q] = 0.0;
     }
     return PDDS_OK;
 }

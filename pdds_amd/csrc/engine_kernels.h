@@ -135,13 +135,15 @@ __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uin
         if (even) {
             const uint32_t call = s.st >> 1;
             Philox4 w = philox4x32_10(call, s.path, knot, 0u, seed, stream);
-            const float sc = 5.9604644775390625e-08f;
-            float u1 = ((float)(w.v[0] >> 8) + 0.5f) * sc;
-            float u2 = ((float)(w.v[1] >> 8) + 0.5f) * sc;
-            float u3 = ((float)(w.v[2] >> 8) + 0.5f) * sc;
-            float u4 = ((float)(w.v[3] >> 8) + 0.5f) * sc;
-            float r1 = RM<float>::sqrt(-2.0f * RM<float>::log(u1));
-            float r2 = RM<float>::sqrt(-2.0f * RM<float>::log(u3));
+            // u = (k + 1/2) 2^-24 as one fma (one rounding of the same exact value as k + 1/2);
+            // -2 ln u = (-2 ln 2) log2 u with the hardware log2
+            const float sc = 5.9604644775390625e-08f, half = 2.98023223876953125e-08f;
+            float u1 = __builtin_fmaf((float)(w.v[0] >> 8), sc, half);
+            float u2 = __builtin_fmaf((float)(w.v[1] >> 8), sc, half);
+            float u3 = __builtin_fmaf((float)(w.v[2] >> 8), sc, half);
+            float u4 = __builtin_fmaf((float)(w.v[3] >> 8), sc, half);
+            float r1 = RM<float>::sqrt(__builtin_amdgcn_logf(u1) * -1.3862943611198906f);
+            float r2 = RM<float>::sqrt(__builtin_amdgcn_logf(u3) * -1.3862943611198906f);
             n1 = r1 * __builtin_amdgcn_cosf(u2);
             n2 = r1 * __builtin_amdgcn_sinf(u2);
             s.pn1 = r2 * __builtin_amdgcn_cosf(u4);

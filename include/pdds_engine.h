@@ -89,6 +89,11 @@ typedef struct {
                                paths, no lane recycling, no persistent queue (the baseline GPU form) */
     int32_t paths_per_lane; /* persistent launches: a wave takes 64 x this many paths from its queue
                                at once (0: 4); results do not depend on it */
+    int32_t plain_step;     /* 0 (default): the control-variate step, xi accumulating
+                               Y (-sigma^T grad u) . dW with the problem's grad u
+                               (Integrators/FKACIntegrator.hpp:46-57); 1: the plain Euler-Maruyama
+                               step (Integrators/FKACIntegrator.hpp:20-29), xi = 0 -- for problems
+                               without a known gradient */
 } pdds_run;
 void pdds_run_defaults(pdds_run *r);
 

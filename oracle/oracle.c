@@ -274,6 +274,10 @@ typedef struct { double x, y, Y, Z, xi; double sig[4], b[2], c, f, grad[2]; } or
 static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
     double g, u;
     orc_pde_eval(pb, s->x, s->y, s->sig, s->b, &s->c, &s->f, s->grad, &g, &u);
+    if (pb->plain_step) {
+        s->grad[0] = 0.0;
+        s->grad[1] = 0.0;
+    }
 }
 
 /* One control-variate Euler-Maruyama step (Integrators/FKACIntegrator.hpp:46-57):

@@ -45,6 +45,8 @@ typedef struct {
     const float *lut;
     int lut_nx, lut_ny, lut_nf;
     double lut_ox, lut_oy, lut_sx, lut_sy;
+    int plain_step;         /* 1: the plain EM step (Integrators/FKACIntegrator.hpp:20-29), no
+                               control variate: grad u taken as 0, so xi stays 0 */
 } orc_problem;
 
 typedef struct {

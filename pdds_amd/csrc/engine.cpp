@@ -48,10 +48,11 @@ pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
 
 bool valid_pde(int pde) { return pde >= PDDS_PDE_MONEGROS && pde <= PDDS_PDE_HELMHOLTZ; }
 
-pdds_status make_params(pdds_engine *e, const pdds_problem *p, PdeParams &pp) {
+pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r, PdeParams &pp) {
     if (!p || !valid_pde(p->pde)) return fail(e, PDDS_ERR_INVALID, "invalid problem descriptor");
     std::memset(&pp, 0, sizeof(pp));
     pp.k2 = p->k2;
+    pp.plain_step = r && r->plain_step ? 1 : 0;
     if (p->pde == PDDS_PDE_VARCOEF_LUT) {
         if (!p->lut || p->lut->nf < 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
         pp.lut = p->lut->view;
@@ -145,7 +146,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
                          int64_t n_vals, double *G_host, double *Gvar_host, AsmOut &out, const double *W_host) {
     auto host_sd = [&](const AsmKnot &k) -> const StencilDesc & { return tabs[k.table]->host_sten[k.stencil]; };
     PdeParams pp;
-    pdds_status st = make_params(e, prob, pp);
+    pdds_status st = make_params(e, prob, r, pp);
     if (st) return st;
     const int S = r->sub_batch > 0 ? r->sub_batch : 32;
     const int wg = r->wg_size > 0 ? r->wg_size : 256;
@@ -781,7 +782,7 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
     const Mesh &M = m->m;
     if (M.any_singular) return fail(e, PDDS_ERR_SINGULAR, "mesh has a singular Psi");
     PdeParams pp;
-    if ((st = make_params(e, p, pp))) return st;
+    if ((st = make_params(e, p, r, pp))) return st;
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     DevTables *T;
     if ((st = mesh_dev_tables(e, m, r->quirk_last_side, &T))) return st;
@@ -1023,7 +1024,7 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
     const Mesh &M = m->m;
     if (M.any_singular) return fail(e, PDDS_ERR_SINGULAR, "mesh has a singular Psi");
     PdeParams pp;
-    if ((st = make_params(e, p, pp))) return st;
+    if ((st = make_params(e, p, base, pp))) return st;
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     DevTables *T;
     if ((st = mesh_dev_tables(e, m, base->quirk_last_side, &T))) return st;
@@ -1104,7 +1105,7 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
     if (st) return st;
     const Mesh &M = m->m;
     PdeParams pp;
-    if ((st = make_params(e, p, pp))) return st;
+    if ((st = make_params(e, p, r, pp))) return st;
     if (M.on_boundary[knot]) {
         *B = host_g(p->pde, M.x0[knot], M.y0[knot], pp);
         *BB = *B * *B;
@@ -1137,7 +1138,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
     if (st) return st;
     if (!(glob[2] > glob[0]) || !(glob[3] > glob[1])) return fail(e, PDDS_ERR_INVALID, "bad rectangle");
     PdeParams pp;
-    if ((st = make_params(e, p, pp))) return st;
+    if ((st = make_params(e, p, r, pp))) return st;
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     StencilDesc d;
     std::memset(&d, 0, sizeof(d));
@@ -1191,7 +1192,7 @@ pdds_status pdds_paths(pdds_engine *e, const pdds_problem *p, const pdds_run *r,
     if (!rect || n < 0 || path0 < 0 || path0 + n > ((int64_t)1 << 32)) return fail(e, PDDS_ERR_INVALID, "bad path range");
     if (!(rect[2] > rect[0]) || !(rect[3] > rect[1])) return fail(e, PDDS_ERR_INVALID, "bad rectangle");
     PdeParams pp;
-    if ((st = make_params(e, p, pp))) return st;
+    if ((st = make_params(e, p, r, pp))) return st;
     if (n == 0) return PDDS_OK;
     if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
     PathsArgs a;

@@ -103,7 +103,7 @@ pdds_status upload(pdds_engine *e, int slot, const std::vector<T> &v, T **out) {
 }
 
 bool valid_pde(int pde);
-pdds_status make_params(pdds_engine *e, const pdds_problem *p, PdeParams &pp);
+pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r, PdeParams &pp);
 double host_g(int pde, double x, double y, const PdeParams &pp);
 pdds_status check_run(pdds_engine *e, const pdds_run *r);
 

@@ -22,6 +22,7 @@ template <class R> struct Coef {
 struct PdeParams {
     double k2;             // HELMHOLTZ (launches without per-knot values)
     LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c
+    int32_t plain_step;    // 1: no control variate (grad u taken as 0, so xi stays 0)
 };
 
 // Monegros Poisson (BVPs/EDPs/Monegros_Poisson_CPP.hpp:4-81): u, grad u, Laplacian
@@ -99,6 +100,11 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Co
             co.s00 = v[0]; co.s01 = 0; co.s10 = 0; co.s11 = v[1];
             co.b0 = v[2]; co.b1 = v[3]; co.c = v[4];
         }
+    }
+    // the plain step: no control variate (Integrators/FKACIntegrator.hpp:20-29)
+    if (P.plain_step) {
+        co.g0 = 0;
+        co.g1 = 0;
     }
 }
 

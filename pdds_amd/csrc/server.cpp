@@ -330,7 +330,7 @@ pdds_status pdds_server_create(pdds_engine *e, const pdds_server_config *cfg, pd
     {
         std::lock_guard<std::mutex> lk(e->mu);
         pdds_status st = check_run(e, &c.run);
-        if (!st) st = make_params(e, &c.problem, s->pp);
+        if (!st) st = make_params(e, &c.problem, &c.run, s->pp);
         if (st) return st;
     }
     pdds_server *sp = s.get();

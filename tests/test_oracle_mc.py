@@ -76,6 +76,26 @@ def test_control_variate_is_mean_zero_and_reduces_variance():
     assert (plain + p["xi"]).var() < 0.05 * plain.var()
 
 
+def test_plain_step_is_the_control_variate_step_without_xi():
+    """plain_step = 1 (the reference's plain EM step): on the same Philox draws every path has
+    the X, Y, Z, exit and step count of the control-variate step (xi only accumulates), xi is
+    exactly 0, and the plain estimate is unbiased with the larger variance the control variate
+    removes."""
+    run = O.run_params(1e-3, 5, threads=8)
+    glob = np.array([-1.0, -1.0, 1.0, 1.0])
+    pcv = O.simulate(O.problem(O.PDE_SINSIN), run, 0.3, -0.2, glob, 0, 0, 20000)
+    ppl = O.simulate(O.problem(O.PDE_SINSIN, plain_step=1), run, 0.3, -0.2, glob, 0, 0, 20000)
+    for k in ("xt", "yt", "Y", "Z", "steps"):
+        assert np.array_equal(pcv[k], ppl[k]), k
+    assert np.all(ppl["xi"] == 0.0) and np.any(pcv["xi"] != 0.0)
+    pb = O.problem(O.PDE_SINSIN)
+    g = np.array([O.pde_eval(pb, a, b)["g"] for a, b in zip(ppl["xt"], ppl["yt"])])
+    score = ppl["Z"] + ppl["Y"] * g
+    u = O.pde_eval(pb, 0.3, -0.2)["u"]
+    assert abs(score.mean() - u) < 4 * score.std() / np.sqrt(len(g)) + 12e-3
+    assert (score + pcv["xi"]).var() < 0.05 * score.var()
+
+
 def torsion_centre():
     """w(0,0) for -Lap w = 1 on [-1,1]^2, w = 0 on the boundary: with
     (1-x^2)/2 = sum_{k odd} a_k cos(k pi x/2), a_k = 16 (-1)^((k-1)/2) / (k pi)^3,

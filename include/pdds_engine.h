@@ -69,6 +69,15 @@ typedef struct {
 } pdds_problem;
 
 /* ---- run parameters ---- */
+/* Step families of the Euler-Maruyama scheme (Integrators/FKACIntegrator.hpp), with mu = -sigma^T
+   grad u / u from the problem's u and grad u: */
+#define PDDS_STEP_CV 0          /* control variate: xi += Y (-sigma^T grad u) . dW
+                                   (FKACIntegrator.hpp:46-57), the default */
+#define PDDS_STEP_PLAIN 1       /* plain step, xi = 0 (FKACIntegrator.hpp:20-29): problems without a
+                                   known gradient */
+#define PDDS_STEP_VR 2          /* variance reduction by a change of drift: Y += Y mu . dW and
+                                   X += (b - sigma mu) h + sigma dW, xi = 0 (FKACIntegrator.hpp:31-44);
+                                   u must not vanish along the paths */
 #define PDDS_FP64 0             /* FP64 path state; bit-exact with the FP64 oracle */
 #define PDDS_FP32 1             /* FP32 path state, Kahan functionals, FP64 cross-path sums */
 typedef struct {
@@ -89,11 +98,7 @@ typedef struct {
                                paths, no lane recycling, no persistent queue (the baseline GPU form) */
     int32_t paths_per_lane; /* persistent launches: a wave takes 64 x this many paths from its queue
                                at once (0: 4); results do not depend on it */
-    int32_t plain_step;     /* 0 (default): the control-variate step, xi accumulating
-                               Y (-sigma^T grad u) . dW with the problem's grad u
-                               (Integrators/FKACIntegrator.hpp:46-57); 1: the plain Euler-Maruyama
-                               step (Integrators/FKACIntegrator.hpp:20-29), xi = 0 -- for problems
-                               without a known gradient */
+    int32_t step;           /* step family (PDDS_STEP_*, default 0) */
 } pdds_run;
 void pdds_run_defaults(pdds_run *r);
 

@@ -274,25 +274,38 @@ typedef struct { double x, y, Y, Z, xi; double sig[4], b[2], c, f, grad[2]; } or
 static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
     double g, u;
     orc_pde_eval(pb, s->x, s->y, s->sig, s->b, &s->c, &s->f, s->grad, &g, &u);
-    if (pb->plain_step) {
+    if (pb->step == 1) {
         s->grad[0] = 0.0;
         s->grad[1] = 0.0;
+    } else if (pb->step == 2) {
+        s->grad[0] = s->grad[0] / g;   /* g = u for every manufactured problem */
+        s->grad[1] = s->grad[1] / g;
     }
 }
 
-/* One control-variate Euler-Maruyama step (Integrators/FKACIntegrator.hpp:46-57):
-   all updates use the old X and Y; Dirichlet everywhere so psi = varphi = 0.
-   Coefficients at the old X are already in s. */
-static inline void orc_step(orc_state *s, double h, double sqrth, double n1, double n2) {
+/* One Euler-Maruyama step: all updates use the old X and Y; Dirichlet everywhere so
+   psi = varphi = 0.  Coefficients at the old X are already in s, with v = -sigma^T grad.
+   Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29):
+   xi += Y v . dW.  VR step (:31-44, grad holds grad u / u so v = mu): Y += c Y h + Y mu . dW
+   and X's drift is b - sigma mu. */
+static inline void orc_step(int step, orc_state *s, double h, double sqrth, double n1, double n2) {
     double i0 = sqrth * n1, i1 = sqrth * n2;
     double v0 = -(s->sig[0] * s->grad[0] + s->sig[2] * s->grad[1]);
     double v1 = -(s->sig[1] * s->grad[0] + s->sig[3] * s->grad[1]);
-    s->xi = s->xi + s->Y * (v0 * i0 + v1 * i1);
+    double dxi = s->Y * (v0 * i0 + v1 * i1);
+    if (step != 2) s->xi = s->xi + dxi;
     s->Z = s->Z + s->f * s->Y * h;
     double Yold = s->Y;
-    s->Y = Yold + s->c * Yold * h;
-    double nx = s->x + (s->b[0] * h + (s->sig[0] * i0 + s->sig[1] * i1));
-    double ny = s->y + (s->b[1] * h + (s->sig[2] * i0 + s->sig[3] * i1));
+    double b0 = s->b[0], b1 = s->b[1];
+    if (step == 2) {
+        s->Y = Yold + (s->c * Yold * h + dxi);
+        b0 = b0 - (s->sig[0] * v0 + s->sig[1] * v1);
+        b1 = b1 - (s->sig[2] * v0 + s->sig[3] * v1);
+    } else {
+        s->Y = Yold + s->c * Yold * h;
+    }
+    double nx = s->x + (b0 * h + (s->sig[0] * i0 + s->sig[1] * i1));
+    double ny = s->y + (b1 * h + (s->sig[2] * i0 + s->sig[3] * i1));
     s->x = nx; s->y = ny;
 }
 
@@ -331,7 +344,7 @@ static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double 
         do {
             double n1, n2;
             orc_normal_pair(run, knot, path, st, &n1, &n2);
-            orc_step(&s, h, sqrth, n1, n2);
+            orc_step(pb->step, &s, h, sqrth, n1, n2);
             ++st;
             in = orc_inside(pb, &s, geom, rect, run->beta, sqrth, &ex, &ey);
         } while (in && st < cap);
@@ -347,7 +360,7 @@ static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double 
             do {
                 double n1, n2;
                 orc_normal_pair(run, knot, path, st + st2, &n1, &n2);
-                orc_step(&s, h, sqrth, n1, n2);
+                orc_step(pb->step, &s, h, sqrth, n1, n2);
                 ++st2;
                 in = orc_inside(pb, &s, geom, rect, 0.0, sqrth, &ex, &ey);
             } while (in && st2 < cap);

@@ -45,8 +45,9 @@ typedef struct {
     const float *lut;
     int lut_nx, lut_ny, lut_nf;
     double lut_ox, lut_oy, lut_sx, lut_sy;
-    int plain_step;         /* 1: the plain EM step (Integrators/FKACIntegrator.hpp:20-29), no
-                               control variate: grad u taken as 0, so xi stays 0 */
+    int step;               /* step family: 0 control variate (Integrators/FKACIntegrator.hpp:46-57),
+                               1 plain, grad u taken as 0 (:20-29), 2 VR, the drift change by
+                               mu = -sigma^T grad u / u (:31-44) */
 } orc_problem;
 
 typedef struct {

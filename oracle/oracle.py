@@ -27,7 +27,7 @@ class Problem(C.Structure):
     _fields_ = [("pde", C.c_int), ("k2", C.c_double), ("lut", _f),
                 ("lut_nx", C.c_int), ("lut_ny", C.c_int), ("lut_nf", C.c_int),
                 ("lut_ox", C.c_double), ("lut_oy", C.c_double),
-                ("lut_sx", C.c_double), ("lut_sy", C.c_double), ("plain_step", C.c_int)]
+                ("lut_sx", C.c_double), ("lut_sy", C.c_double), ("step", C.c_int)]
 
 
 class Run(C.Structure):
@@ -121,11 +121,12 @@ def sector_dist(g, x, y):
     return (d, o[0], o[1], o[2], o[3], int(k[0]))
 
 
-def problem(pde, k2=0.0, lut=None, plain_step=0):
+def problem(pde, k2=0.0, lut=None, step=0):
+    """step: 0 control variate, 1 plain, 2 VR (the engine's pdds_run.step values)."""
     pb = Problem()
     pb.pde = pde
     pb.k2 = k2
-    pb.plain_step = plain_step
+    pb.step = step
     if lut is not None:
         pb._lut_keep = lut["coef"]
         pb.lut = ptr(lut["coef"], _f)

@@ -52,7 +52,7 @@ pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r
     if (!p || !valid_pde(p->pde)) return fail(e, PDDS_ERR_INVALID, "invalid problem descriptor");
     std::memset(&pp, 0, sizeof(pp));
     pp.k2 = p->k2;
-    pp.plain_step = r && r->plain_step ? 1 : 0;
+    pp.step = r ? r->step : STEP_CV;
     if (p->pde == PDDS_PDE_VARCOEF_LUT) {
         if (!p->lut || p->lut->nf < 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
         pp.lut = p->lut->view;
@@ -80,6 +80,9 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
         return fail(e, PDDS_ERR_INVALID, "negative launch parameter");
     if (r->paths_per_lane < 0 || r->paths_per_lane > 64)
         return fail(e, PDDS_ERR_INVALID, "paths_per_lane must be in [0, 64]");
+    static_assert(PDDS_STEP_CV == STEP_CV && PDDS_STEP_PLAIN == STEP_PLAIN && PDDS_STEP_VR == STEP_VR,
+                  "step family values");
+    if (r->step < PDDS_STEP_CV || r->step > PDDS_STEP_VR) return fail(e, PDDS_ERR_INVALID, "bad step family");
     return PDDS_OK;
 }
 

@@ -5,15 +5,22 @@
 
 namespace pdds {
 
+template <int PDE, int GEOM, int FAM>
+static hipError_t path_f32_gf(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
+    if (!persistent)
+        hipLaunchKernelGGL((plain_kernel<float, PDE, GEOM, FAM>), dim3(grid), dim3(wg), 0, s, a);
+    else if (mixed)
+        hipLaunchKernelGGL((path_kernel<float, PDE, true, GEOM, FAM>), dim3(grid), dim3(wg), 0, s, a);
+    else
+        hipLaunchKernelGGL((path_kernel<float, PDE, false, GEOM, FAM>), dim3(grid), dim3(wg), 0, s, a);
+    return hipGetLastError();
+}
+
+// the default control-variate step has kernels of its own, without a family test in the loop
 template <int PDE, int GEOM>
 static hipError_t path_f32_g(bool persistent, bool mixed, int grid, int wg, hipStream_t s, const PathKernelArgs &a) {
-    if (!persistent)
-        hipLaunchKernelGGL((plain_kernel<float, PDE, GEOM>), dim3(grid), dim3(wg), 0, s, a);
-    else if (mixed)
-        hipLaunchKernelGGL((path_kernel<float, PDE, true, GEOM>), dim3(grid), dim3(wg), 0, s, a);
-    else
-        hipLaunchKernelGGL((path_kernel<float, PDE, false, GEOM>), dim3(grid), dim3(wg), 0, s, a);
-    return hipGetLastError();
+    return a.pp.step == STEP_CV ? path_f32_gf<PDE, GEOM, STEP_CV>(persistent, mixed, grid, wg, s, a)
+                                : path_f32_gf<PDE, GEOM, STEP_ANY>(persistent, mixed, grid, wg, s, a);
 }
 
 template <int PDE>
@@ -35,15 +42,21 @@ hipError_t launch_path_f32(int pde, int geom, bool persistent, bool mixed, int g
     }
 }
 
-template <bool MIXED, int GEOM>
-static hipError_t path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+template <bool MIXED, int GEOM, int FAM>
+static hipError_t path_tiled_f32_f(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float, MIXED, GEOM>,
+        hipError_t e = hipFuncSetAttribute((const void *)path_kernel_tiled<float, MIXED, GEOM, FAM>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED, GEOM>), dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<float>()), lds, s, a);
     return hipGetLastError();
+}
+
+template <bool MIXED, int GEOM>
+static hipError_t path_tiled_f32(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
+    return a.pp.step == STEP_CV ? path_tiled_f32_f<MIXED, GEOM, STEP_CV>(grid, lds, s, a)
+                                : path_tiled_f32_f<MIXED, GEOM, STEP_ANY>(grid, lds, s, a);
 }
 
 hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
@@ -54,7 +67,10 @@ hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hip
 
 template <int PDE>
 static hipError_t paths_f32(int grid, int wg, hipStream_t s, const PathsArgs &a) {
-    hipLaunchKernelGGL((paths_kernel<float, PDE>), dim3(grid), dim3(wg), 0, s, a);
+    if (a.pp.step == STEP_CV)
+        hipLaunchKernelGGL((paths_kernel<float, PDE, STEP_CV>), dim3(grid), dim3(wg), 0, s, a);
+    else
+        hipLaunchKernelGGL((paths_kernel<float, PDE, STEP_ANY>), dim3(grid), dim3(wg), 0, s, a);
     return hipGetLastError();
 }
 

@@ -95,26 +95,37 @@ template <class R> struct PathState {
     uint32_t path;     // path index within its knot
 };
 
-// One step of the control-variate EM scheme; coefficients at the old X are in co.
-template <class R>
-__device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h, R sqrth, R n1, R n2) {
+// One EM step; coefficients at the old X are in co, every update uses the old X and Y.
+// Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29; the plain
+// step's g is 0): xi += Y v . dW with v = -sigma^T g.  VR step (FKACIntegrator.hpp:31-44; g is
+// grad u / u, so v is mu): Y += Y mu . dW and the drift of X loses sigma mu; xi is not used.
+template <class R, int FAM>
+__device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h, R sqrth, R n1, R n2, int fam) {
     R i0 = sqrth * n1, i1 = sqrth * n2;
     R v0 = -(co.s00 * co.g0 + co.s10 * co.g1);
     R v1 = -(co.s01 * co.g0 + co.s11 * co.g1);
     R dxi = s.Y * (v0 * i0 + v1 * i1);
     R dz = co.f * s.Y * h;
+    const bool vr = FAM != STEP_CV && fam == STEP_VR;
     if constexpr (sizeof(R) == 8) {
-        s.xi = s.xi + dxi;
+        if (!vr) s.xi = s.xi + dxi;
         s.Z = s.Z + dz;
     } else {
         // Kahan-compensated functionals
-        R yk = dxi - s.xc; R tk = s.xi + yk; s.xc = (tk - s.xi) - yk; s.xi = tk;
+        if (!vr) { R yk = dxi - s.xc; R tk = s.xi + yk; s.xc = (tk - s.xi) - yk; s.xi = tk; }
         R yz = dz - s.zc; R tz = s.Z + yz; s.zc = (tz - s.Z) - yz; s.Z = tz;
     }
     R Yold = s.Y;
-    s.Y = Yold + co.c * Yold * h;
-    R nx = s.x + (co.b0 * h + (co.s00 * i0 + co.s01 * i1));
-    R ny = s.y + (co.b1 * h + (co.s10 * i0 + co.s11 * i1));
+    R b0 = co.b0, b1 = co.b1;
+    if (vr) {
+        s.Y = Yold + (co.c * Yold * h + dxi);
+        b0 = b0 - (co.s00 * v0 + co.s01 * v1);
+        b1 = b1 - (co.s10 * v0 + co.s11 * v1);
+    } else {
+        s.Y = Yold + co.c * Yold * h;
+    }
+    R nx = s.x + (b0 * h + (co.s00 * i0 + co.s01 * i1));
+    R ny = s.y + (b1 * h + (co.s10 * i0 + co.s11 * i1));
     s.x = nx;
     s.y = ny;
 }
@@ -225,7 +236,7 @@ template <int GEOM> constexpr int domain_len() { return GEOM == GeomDisc ? 6 : 4
 // ----------------------------------------------------------------------------
 // MIXED: the knots of the launch differ in Philox stream or Helmholtz k2 (serving rounds), so
 // each lane carries its knot's; otherwise both are the launch's and stay in scalar registers.
-template <class R, int PDE, bool MIXED, int GEOM>
+template <class R, int PDE, bool MIXED, int GEOM, int FAM>
 struct Walker {
     PathState<R> s;
     Coef<R> co;            // coefficients at the current X
@@ -281,7 +292,7 @@ struct Walker {
         if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) == 0);
         {
             PathState<R> t = s;
-            em_step<R>(t, co, h, sqrth, pn1, pn2);
+            em_step<R, FAM>(t, co, h, sqrth, pn1, pn2, a.pp.step);
             s.x = stepping ? t.x : s.x;
             s.y = stepping ? t.y : s.y;
             s.Y = stepping ? t.Y : s.Y;
@@ -296,7 +307,7 @@ struct Walker {
         R ex, ey, nx, ny;
         int kind;
         const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        pde_coef_t<PDE, R, TILED>(s.x, s.y, a.pp, T, kk2, co);
+        pde_coef_t<PDE, R, TILED, FAM>(s.x, s.y, a.pp, T, kk2, co);
         if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
@@ -360,11 +371,11 @@ struct WavePool {
 // ----------------------------------------------------------------------------
 // Persistent path kernel: one path per lane, lanes refilled from XCD-local queues
 // ----------------------------------------------------------------------------
-template <class R, int PDE, bool MIXED, int GEOM>
+template <class R, int PDE, bool MIXED, int GEOM, int FAM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
     const int lane = __lane_id();
     const LutTile none{nullptr, 0, 0, 0, 0};
-    Walker<R, PDE, MIXED, GEOM> w;
+    Walker<R, PDE, MIXED, GEOM, FAM> w;
     w.init();
     bool done = false;
     // The wave starts on its XCD's queue, so the paths (and LUT texels) of one XCD's L2
@@ -400,7 +411,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
 // its chunked relaunches, Solvers/SolveLoop.cuh:20-887): one work-item per path of the batch,
 // each stepping its own path to the exit; a wave lasts as long as its slowest path.
 // ----------------------------------------------------------------------------
-template <class R, int PDE, int GEOM>
+template <class R, int PDE, int GEOM, int FAM>
 __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
     const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gg >= a.total) return;
@@ -417,17 +428,17 @@ __global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
     s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
     s.st = 0;
     Coef<R> co;
-    pde_coef_t<PDE, R, false>(s.x, s.y, a.pp, none, k2, co);
+    pde_coef_t<PDE, R, false, FAM>(s.x, s.y, a.pp, none, k2, co);
     R ex, ey;
     int kind;
     bool go, capped;
     do {
         R n1, n2, nx, ny;
         draw_normals<R>(s, a.seed, kd.stream, kd.knot_id, n1, n2, (s.st & 1) == 0);
-        em_step<R>(s, co, h, sqrth, n1, n2);
+        em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
         s.st += 1;
         const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-        pde_coef_t<PDE, R, false>(s.x, s.y, a.pp, none, k2, co);
+        pde_coef_t<PDE, R, false, FAM>(s.x, s.y, a.pp, none, k2, co);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
@@ -456,13 +467,13 @@ __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) 
     return lo;
 }
 
-template <class R, bool MIXED, int GEOM>
+template <class R, bool MIXED, int GEOM, int FAM>
 __global__ void __launch_bounds__(tiled_wg<R>()) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE_VARCOEF_LUT>())))
 path_kernel_tiled(PathKernelArgs a) {
     extern __shared__ float4 tile[];
     __shared__ int s_patch;
     const int lane = __lane_id();
-    Walker<R, PDE_VARCOEF_LUT, MIXED, GEOM> w;
+    Walker<R, PDE_VARCOEF_LUT, MIXED, GEOM, FAM> w;
     w.init();
     // first patch: the one holding this work-group's share of the batch's path range
     int p = find_patch(a.patches, a.n_patches, (int64_t)blockIdx.x * a.total / (int64_t)gridDim.x);
@@ -526,7 +537,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);
 // ----------------------------------------------------------------------------
 // Per-path kernel (MCinCUDA shape): one path per lane, records written out.
 // ----------------------------------------------------------------------------
-template <class R, int PDE>
+template <class R, int PDE, int FAM>
 __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -538,7 +549,7 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
     s.Y = (R)1; s.Z = (R)0; s.xi = (R)0; s.zc = (R)0; s.xc = (R)0;
     s.st = 0;
     Coef<R> co;
-    pde_coef<PDE, R>(s.x, s.y, a.pp, co);
+    pde_coef<PDE, R, FAM>(s.x, s.y, a.pp, co);
     R ex = s.x, ey = s.y;
     for (int phase = 0; phase < (a.phase2 ? 2 : 1); ++phase) {
         const R beta = phase == 0 ? (R)a.beta : (R)0;
@@ -548,10 +559,10 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
             R n1, n2, nx, ny;
             int kind;
             draw_normals<R>(s, a.seed, a.stream, a.knot, n1, n2, (s.st & 1) == 0);
-            em_step<R>(s, co, h, sqrth, n1, n2);
+            em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
             s.st += 1;
             R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
-            pde_coef<PDE, R>(s.x, s.y, a.pp, co);
+            pde_coef<PDE, R, FAM>(s.x, s.y, a.pp, co);
             R ga = nx * co.s00 + ny * co.s10;
             R gb = nx * co.s01 + ny * co.s11;
             R nrm = RM<R>::sqrt(ga * ga + gb * gb);

@@ -11,6 +11,9 @@
 namespace pdds {
 
 enum PdeId { PDE_MONEGROS = 0, PDE_SINSIN = 1, PDE_VARCOEF = 2, PDE_VARCOEF_LUT = 3, PDE_HELMHOLTZ = 4 };
+// Step families, the values of pdds_run.step.  Kernels are compiled for STEP_CV alone (the
+// default step, no family test in the loop) and for STEP_ANY (the family read at run time).
+enum StepFamily { STEP_ANY = -1, STEP_CV = 0, STEP_PLAIN = 1, STEP_VR = 2 };
 
 template <class R> struct Coef {
     R s00, s01, s10, s11;  // sigma, row-major
@@ -22,7 +25,7 @@ template <class R> struct Coef {
 struct PdeParams {
     double k2;             // HELMHOLTZ (launches without per-knot values)
     LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c
-    int32_t plain_step;    // 1: no control variate (grad u taken as 0, so xi stays 0)
+    int32_t step;          // StepFamily: PLAIN takes grad u as 0, VR hands out grad u / u
 };
 
 // Monegros Poisson (BVPs/EDPs/Monegros_Poisson_CPP.hpp:4-81): u, grad u, Laplacian
@@ -49,10 +52,12 @@ PDDS_HD void monegros(R x, R y, R &u, R &gx, R &gy, R &lap) {
     lap = dxx + dyy;
 }
 
+template <int PDE, class R> PDDS_HD R pde_g(R x, R y, const PdeParams &P);
+
 // TILED: the coefficient LUT's fields 0..3 are read from the LDS tile T where it covers X.
 // k2: the Helmholtz parameter of this path's problem (per knot, so one launch can carry
 // requests with different k^2).
-template <int PDE, class R, bool TILED>
+template <int PDE, class R, bool TILED, int FAM = STEP_CV>
 PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Coef<R> &co) {
     (void)T;
     if constexpr (PDE == PDE_MONEGROS) {
@@ -101,17 +106,24 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Co
             co.b0 = v[2]; co.b1 = v[3]; co.c = v[4];
         }
     }
-    // the plain step: no control variate (Integrators/FKACIntegrator.hpp:20-29)
-    if (P.plain_step) {
-        co.g0 = 0;
-        co.g1 = 0;
+    if constexpr (FAM != STEP_CV) {
+        if (P.step == STEP_PLAIN) {
+            // no control variate (Integrators/FKACIntegrator.hpp:20-29)
+            co.g0 = 0;
+            co.g1 = 0;
+        } else if (P.step == STEP_VR) {
+            // grad u / u, so that the step's -sigma^T g is mu (Integrators/FKACIntegrator.hpp:31-44)
+            const R uu = pde_g<PDE, R>(x, y, P);     // u (every shipped problem has g = u)
+            co.g0 = co.g0 / uu;
+            co.g1 = co.g1 / uu;
+        }
     }
 }
 
-template <int PDE, class R>
+template <int PDE, class R, int FAM = STEP_CV>
 PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
     const LutTile none{nullptr, 0, 0, 0, 0};
-    pde_coef_t<PDE, R, false>(x, y, P, none, (R)P.k2, co);
+    pde_coef_t<PDE, R, false, FAM>(x, y, P, none, (R)P.k2, co);
 }
 
 // Dirichlet data g = u (all shipped problems are manufactured solutions)

@@ -77,14 +77,14 @@ def test_control_variate_is_mean_zero_and_reduces_variance():
 
 
 def test_plain_step_is_the_control_variate_step_without_xi():
-    """plain_step = 1 (the reference's plain EM step): on the same Philox draws every path has
+    """step = 1 (the reference's plain EM step): on the same Philox draws every path has
     the X, Y, Z, exit and step count of the control-variate step (xi only accumulates), xi is
     exactly 0, and the plain estimate is unbiased with the larger variance the control variate
     removes."""
     run = O.run_params(1e-3, 5, threads=8)
     glob = np.array([-1.0, -1.0, 1.0, 1.0])
     pcv = O.simulate(O.problem(O.PDE_SINSIN), run, 0.3, -0.2, glob, 0, 0, 20000)
-    ppl = O.simulate(O.problem(O.PDE_SINSIN, plain_step=1), run, 0.3, -0.2, glob, 0, 0, 20000)
+    ppl = O.simulate(O.problem(O.PDE_SINSIN, step=1), run, 0.3, -0.2, glob, 0, 0, 20000)
     for k in ("xt", "yt", "Y", "Z", "steps"):
         assert np.array_equal(pcv[k], ppl[k]), k
     assert np.all(ppl["xi"] == 0.0) and np.any(pcv["xi"] != 0.0)
@@ -94,6 +94,25 @@ def test_plain_step_is_the_control_variate_step_without_xi():
     u = O.pde_eval(pb, 0.3, -0.2)["u"]
     assert abs(score.mean() - u) < 4 * score.std() / np.sqrt(len(g)) + 12e-3
     assert (score + pcv["xi"]).var() < 0.05 * score.var()
+
+
+def test_vr_step_is_unbiased_and_nearly_variance_free():
+    """step = 2 (the reference's VR step, drift change by mu = -sigma^T grad u / u): with the exact
+    u the weight Y carries the whole estimate, so the VR score Z + Y g is unbiased and its variance
+    is a small fraction of the plain step's (zero in the limit h -> 0); xi stays 0."""
+    run = O.run_params(1e-3, 11, threads=8)
+    glob = np.array([-1.0, -1.0, 1.0, 1.0])
+    out = {}
+    for step in (1, 2):
+        pb = O.problem(O.PDE_VARCOEF, step=step)
+        p = O.simulate(pb, run, 0.3, -0.2, glob, 0, 0, 8000)
+        g = np.array([O.pde_eval(pb, a, b)["g"] for a, b in zip(p["xt"], p["yt"])])
+        out[step] = (p["Z"] + p["Y"] * g, p)
+    vr, p = out[2]
+    u = O.pde_eval(O.problem(O.PDE_VARCOEF), 0.3, -0.2)["u"]
+    assert np.all(p["xi"] == 0.0)
+    assert abs(vr.mean() - u) < 4 * vr.std() / np.sqrt(len(vr)) + 12e-3
+    assert vr.var() < 0.05 * out[1][0].var()
 
 
 def torsion_centre():

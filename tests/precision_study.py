@@ -146,7 +146,8 @@ def main():
         with open(args.out, "w") as f:
             json.dump({"workload": "2x2 subdomains x 16 knots/interface on [-1,1]^2, u = sin(pi x) sin(pi y) "
                                    "+ x^2 + y^2, seed 42; FP32 engine vs FP64-stepped oracle, same Philox words",
-                       "rows": rows}, f, indent=1)
+                       "rows": rows, "command": "python3 tests/precision_study.py " + " ".join(sys.argv[1:])},
+                      f, indent=1)
 
 
 if __name__ == "__main__":

@@ -101,12 +101,6 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 // are in flight; Phi is kept [entry][sub-batch] so that the Psi^-T pass reads a row of kNBG
 // values with two 16-byte LDS reads per pair of sub-batches.
 // EQ: the launch carries reference values (ReduceArgs::wref) and also sums the knot equations
-// x / n for a sub-batch of n paths: a multiply by the exact reciprocal for a full sub-batch of a
-// power-of-two size (the same rounded value as the division), a division otherwise
-__device__ __forceinline__ double per_batch(double x, int n, int S, double invS) {
-    return (n == S && invS != 0.0) ? x * invS : x / (double)n;
-}
-
 template <bool EQ>
 __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(4))) reduce_kernel(ReduceArgs a) {
     extern __shared__ __align__(16) double smem[];
@@ -129,8 +123,6 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     const int32_t *rmeta = a.rec.meta + g0;
     const int S = a.S;
     const int GS = kNBG * S;
-    // 1/S when S is a power of two (then x * (1/S) == x / S exactly), else 0
-    const double invS = (S & (S - 1)) == 0 ? 1.0 / (double)S : 0.0;
     double *Phi = smem;                                     // [ne_cap][kNBG]
     double *accS = smem + kNBG * a.ne_cap;
     double *accQ = accS + a.nnz_cap;
@@ -288,7 +280,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 const int p0 = (g + b) * S;
                 const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
                 sum = sum + G[b];
-                q2 = q2 + per_batch(G[b] * G[b], p1 - p0, S, invS);
+                q2 = q2 + (G[b] * G[b]) / (double)(p1 - p0);
             }
             accS[c] = sum;
             accQ[c] = q2;
@@ -318,7 +310,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                     const int p1 = (p0 + S) < it.n ? (p0 + S) : it.n;
                     const double R = s_bsb[b] - T;
                     Rs = Rs + R;
-                    RQs = RQs + per_batch(R * R, p1 - p0, S, invS);
+                    RQs = RQs + (R * R) / (double)(p1 - p0);
                 }
         }
         __syncthreads();

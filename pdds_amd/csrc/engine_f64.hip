@@ -97,7 +97,7 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 // sub-batches is staged in LDS with, per (sub-batch, side), the list of its paths that exit
 // through that side in path order (built with wave ballots).  A thread then owns one stencil
 // entry e and, for each sub-batch b, adds Y phi_e(X) over the list of e's side in path order
-// (the order of the per-exit loop it replaces), two exits per iteration so that two exp chains
+// (the order of the per-exit loop it replaces), four exits per iteration so that four exp chains
 // are in flight; Phi is kept [entry][sub-batch] so that the Psi^-T pass reads a row of kNBG
 // values with two 16-byte LDS reads per pair of sub-batches.
 // EQ: the launch carries reference values (ReduceArgs::wref) and also sums the knot equations
@@ -213,14 +213,18 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 const int pb = b * S;
                 double acc = 0.0;
                 int l = 0;
-                for (; l + 1 < nl; l += 2) {
-                    const int pa = pb + L[l], pc = pb + L[l + 1];
+                for (; l + 3 < nl; l += 4) {
+                    const int pa = pb + L[l], pc = pb + L[l + 1], pd = pb + L[l + 2], pe = pb + L[l + 3];
                     const double fa = rbf_tab(pxe, pye, gx[pa], gy[pa], ic2, s_exp2);
                     const double fc = rbf_tab(pxe, pye, gx[pc], gy[pc], ic2, s_exp2);
+                    const double fd = rbf_tab(pxe, pye, gx[pd], gy[pd], ic2, s_exp2);
+                    const double fe = rbf_tab(pxe, pye, gx[pe], gy[pe], ic2, s_exp2);
                     acc = __builtin_fma(gY[pa], fa, acc);
                     acc = __builtin_fma(gY[pc], fc, acc);
+                    acc = __builtin_fma(gY[pd], fd, acc);
+                    acc = __builtin_fma(gY[pe], fe, acc);
                 }
-                if (l < nl) {
+                for (; l < nl; ++l) {
                     const int pa = pb + L[l];
                     acc = __builtin_fma(gY[pa], rbf_tab(pxe, pye, gx[pa], gy[pa], ic2, s_exp2), acc);
                 }

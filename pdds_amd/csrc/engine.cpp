@@ -496,12 +496,18 @@ pdds_status pdds_engine_create(int device, pdds_engine **out) {
     pdds_engine *e = new pdds_engine();
     e->device = device;
     e->bufs.resize(B_NSLOTS);
+    // on a failure part way, pdds_engine_destroy releases what was created (members start null)
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete e;
+        e->stream = nullptr;
+        pdds_engine_destroy(e);
         return PDDS_ERR_DEVICE;
     }
     for (auto &ev : e->ev)
-        if (hipEventCreate(&ev) != hipSuccess) { delete e; return PDDS_ERR_DEVICE; }
+        if (hipEventCreate(&ev) != hipSuccess) {
+            ev = nullptr;
+            pdds_engine_destroy(e);
+            return PDDS_ERR_DEVICE;
+        }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         e->n_cu = prop.multiProcessorCount;

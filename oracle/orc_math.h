@@ -9,9 +9,9 @@
  *   exp   : x = k ln2 + r (two-part ln2, Cody-Waite), Taylor series of e^r to
  *           degree 14, scaling by 2^k through the exponent field.
  *   log   : x = m 2^e with m in [1/sqrt2, sqrt2), log m = 2 atanh(s),
- *           s = (m-1)/(m+1), odd series to s^27.
+ *           s = (m-1)/(m+1), odd series to s^23.
  *   sinpi/cospi : exact reduction x = k/2 + r, |r| <= 1/4, Taylor series of
- *           sin and cos of pi*r to degree 21 / 22.
+ *           sin and cos of pi*r to degree 17 / 18.
  *   sin/cos : sinpi/cospi of x/pi with x/pi formed from a two-part 1/pi.
  *   tanh  : expm1(2x) / (expm1(2x) + 2), expm1 by Taylor for |y| <= 1/2.
  * The reference (BVPs/EDPs/Monegros_Poisson_CPP.hpp:12-59,
@@ -158,9 +158,8 @@ static inline double orc_log(double x) {
     if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
     double s = (m - 1.0) / (m + 1.0);
     double s2 = s * s;
-    double p = 0.07407407407407407;         /* 2/27 */
-    p = fma(p, s2, 0.08);                    /* 2/25 */
-    p = fma(p, s2, 0.08695652173913043);     /* 2/23 */
+    /* s^2 <= 0.0295: the atanh series terms past s^23 are below 1e-2 ulp */
+    double p = 0.08695652173913043;          /* 2/23 */
     p = fma(p, s2, 0.09523809523809523);     /* 2/21 */
     p = fma(p, s2, 0.10526315789473684);     /* 2/19 */
     p = fma(p, s2, 0.11764705882352941);     /* 2/17 */
@@ -181,9 +180,8 @@ static inline double orc_log(double x) {
 static inline void orc_sincospi_kernel(double r, double *s, double *c) {
     double y = fma(r, ORC_PI, r * ORC_PI_LO);
     double y2 = y * y;
-    double ps = 1.9572941063391263e-20;      /* 1/21! */
-    ps = fma(ps, y2, -8.22063524662433e-18);
-    ps = fma(ps, y2, 2.8114572543455206e-15);
+    /* |y| <= pi/4: the Taylor terms past y^17 (sin) and y^18 (cos) are below 1e-3 ulp */
+    double ps = 2.8114572543455206e-15;      /* 1/17! */
     ps = fma(ps, y2, -7.647163731819816e-13);
     ps = fma(ps, y2, 1.6059043836821613e-10);
     ps = fma(ps, y2, -2.505210838544172e-08);
@@ -192,9 +190,7 @@ static inline void orc_sincospi_kernel(double r, double *s, double *c) {
     ps = fma(ps, y2, 0.008333333333333333);
     ps = fma(ps, y2, -0.16666666666666666);
     *s = fma(ps * y2, y, y);
-    double pc = -8.896791392450574e-22;      /* -1/22! */
-    pc = fma(pc, y2, 4.110317623312165e-19);
-    pc = fma(pc, y2, -1.5619206968586225e-16);
+    double pc = -1.5619206968586225e-16;     /* -1/18! */
     pc = fma(pc, y2, 4.779477332387385e-14);
     pc = fma(pc, y2, -1.1470745597729725e-11);
     pc = fma(pc, y2, 2.08767569878681e-09);

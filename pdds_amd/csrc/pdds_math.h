@@ -5,8 +5,8 @@
 // compiled with -ffp-contract=off), so every FP64 path is a fixed function of
 // its Philox stream.  Recipes (textbook): exp by two-part ln2 reduction and
 // the Taylor series of e^r to degree 14; log by 2 atanh((m-1)/(m+1)) to
-// s^27; sinpi/cospi by exact reduction to |r| <= 1/4 and Taylor series to
-// degree 21/22; sin/cos through a two-part 1/pi; tanh through expm1.
+// s^23; sinpi/cospi by exact reduction to |r| <= 1/4 and Taylor series to
+// degree 17/18; sin/cos through a two-part 1/pi; tanh through expm1.
 // FP32: the CDNA hardware transcendentals (v_exp_f32, v_log_f32, v_sin_f32,
 // v_cos_f32 take base-2 / revolution arguments) and v_rcp_f32.
 //
@@ -204,9 +204,8 @@ PDDS_HD double dlog(double x) {
     if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
     double s = (m - 1.0) / (m + 1.0);
     double s2 = s * s;
-    double p = 0.07407407407407407;
-    p = __builtin_fma(p, s2, 0.08);
-    p = __builtin_fma(p, s2, 0.08695652173913043);
+    // s^2 <= 0.0295: the atanh series terms past s^23 are below 1e-2 ulp
+    double p = 0.08695652173913043;
     p = __builtin_fma(p, s2, 0.09523809523809523);
     p = __builtin_fma(p, s2, 0.10526315789473684);
     p = __builtin_fma(p, s2, 0.11764705882352941);
@@ -226,9 +225,8 @@ PDDS_HD double dlog(double x) {
 PDDS_HD void sincospi_kernel(double r, double &s, double &c) {
     double y = __builtin_fma(r, kPi, r * kPiLo);
     double y2 = y * y;
-    double ps = 1.9572941063391263e-20;
-    ps = __builtin_fma(ps, y2, -8.22063524662433e-18);
-    ps = __builtin_fma(ps, y2, 2.8114572543455206e-15);
+    // |y| <= pi/4: the Taylor terms past y^17 (sin) and y^18 (cos) are below 1e-3 ulp
+    double ps = 2.8114572543455206e-15;
     ps = __builtin_fma(ps, y2, -7.647163731819816e-13);
     ps = __builtin_fma(ps, y2, 1.6059043836821613e-10);
     ps = __builtin_fma(ps, y2, -2.505210838544172e-08);
@@ -237,9 +235,7 @@ PDDS_HD void sincospi_kernel(double r, double &s, double &c) {
     ps = __builtin_fma(ps, y2, 0.008333333333333333);
     ps = __builtin_fma(ps, y2, -0.16666666666666666);
     s = __builtin_fma(ps * y2, y, y);
-    double pc = -8.896791392450574e-22;
-    pc = __builtin_fma(pc, y2, 4.110317623312165e-19);
-    pc = __builtin_fma(pc, y2, -1.5619206968586225e-16);
+    double pc = -1.5619206968586225e-16;
     pc = __builtin_fma(pc, y2, 4.779477332387385e-14);
     pc = __builtin_fma(pc, y2, -1.1470745597729725e-11);
     pc = __builtin_fma(pc, y2, 2.08767569878681e-09);

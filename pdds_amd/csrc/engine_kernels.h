@@ -411,8 +411,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
 // its chunked relaunches, Solvers/SolveLoop.cuh:20-887): one work-item per path of the batch,
 // each stepping its own path to the exit; a wave lasts as long as its slowest path.
 // ----------------------------------------------------------------------------
+// FP64 control-variate plain kernels fit three waves per SIMD without spilling (on cfg2 that runs
+// them 23 % faster than the compiler's own two-wave allocation); the run-time family variants
+// would spill there and keep the default
+template <class R, int FAM> constexpr int plain_waves() { return sizeof(R) == 8 && FAM == STEP_CV ? 3 : 1; }
+
 template <class R, int PDE, int GEOM, int FAM>
-__global__ void __launch_bounds__(256) plain_kernel(PathKernelArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_waves<R, FAM>()))) plain_kernel(PathKernelArgs a) {
     const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gg >= a.total) return;
     const KnotDesc &kd = a.knots[find_knot(a.knots, a.n_knots, gg)];

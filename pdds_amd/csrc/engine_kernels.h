@@ -371,8 +371,14 @@ struct WavePool {
 // ----------------------------------------------------------------------------
 // Persistent path kernel: one path per lane, lanes refilled from XCD-local queues
 // ----------------------------------------------------------------------------
+// the run-time step-family variants carry the VR branch: one wave less (where more than two),
+// so that they do not spill
+template <class R, int PDE, int FAM> constexpr int path_waves_f() {
+    return FAM == STEP_CV || path_waves<R, PDE>() < 3 ? path_waves<R, PDE>() : path_waves<R, PDE>() - 1;
+}
+
 template <class R, int PDE, bool MIXED, int GEOM, int FAM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE>()))) path_kernel(PathKernelArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves_f<R, PDE, FAM>()))) path_kernel(PathKernelArgs a) {
     const int lane = __lane_id();
     const LutTile none{nullptr, 0, 0, 0, 0};
     Walker<R, PDE, MIXED, GEOM, FAM> w;

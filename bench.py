@@ -12,16 +12,62 @@ K timed assemblies (inputs resident on the device).  Extra fields report the pla
 form, the FP32-path engine and the RMS interface error of each.
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
-import platform
 import sys
 import time
 
-import numpy as np
+# the CPU baseline's OpenMP threads: one per core, packed close together (read by the OpenMP
+# runtime when the oracle library is first loaded, so set before anything loads it)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+
+def kernel_source_hash():
+    """SHA-256 (16 hex digits) of the engine's sources and build flags: a traffic figure measured
+    on another build is not reported for this one."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "pdds_amd", "csrc", "*"))) + [os.path.join(ROOT, "pdds_amd", "Makefile")]
+    for p in files:
+        h.update(os.path.basename(p).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def cpu_info():
+    """Model, sockets, physical cores and the logical CPUs this process may run on."""
+    model, phys, sockets = None, set(), set()
+    cur = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + [""]:
+                if not line.strip():
+                    if cur:
+                        phys.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                        sockets.add(cur.get("physical id", "0"))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name" and model is None:
+                    model = v.strip()
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"model": model, "sockets": len(sockets), "physical_cores": len(phys), "logical_cpus": os.cpu_count(),
+            "cpus_available": affinity, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
 
 # Algorithmic FP64 operation count per path-step of the LUT configuration (+,-,x = 1,
 # fma = 2): LUT 264 (coordinates 6, weights 18, 5 fields x 40 + row sums), grad u 2, xi 7,
@@ -38,7 +84,9 @@ def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
     seconds of stepping have elapsed."""
     from oracle import oracle as O
     from pdds_amd.workloads import varcoef_fields
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cpu = cpu_info()
+    # one thread per physical core available to this process (OMP_NUM_THREADS if set)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(cpu["physical_cores"] or 1, cpu["cpus_available"] or 1)
     if prob_name == "varcoef_lut":
         n = 512
         xs = np.linspace(-1, 1, n)
@@ -90,7 +138,7 @@ def cpu_baseline(mesh, prob_name, n_paths_ref, h, seed, budget_s=15.0):
             "one_thread_path_steps_per_s": rate1,
             "reduction_s_per_path_one_thread": {"knot": k, "stencil_entries": int(np.sum(st["msz"])),
                                                 "batched_engine_order": t_red, "per_exit_reference_form": t_faith},
-            "cpu": platform.processor() or platform.machine()}
+            "cpu": cpu}
 
 
 def copy_calibration(n=1 << 30, reps=3):
@@ -110,6 +158,77 @@ def copy_calibration(n=1 << 30, reps=3):
     hip.hipFree(a)
     hip.hipFree(b)
     print(json.dumps({"copy_calibration": {"bytes_read_per_copy": n, "bytes_written_per_copy": n, "copies": reps}}))
+
+
+TRAFFIC_RUNS = {
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc -o fetch -- "
+             "python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc -o write -- "
+             "python3 bench.py --steps 1 --warmup 0 --no-extras --no-cpu-baseline",
+    "cfetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc -o cfetch -- "
+              "python3 bench.py --copy-calibration",
+    "cwrite": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc -o cwrite -- "
+              "python3 bench.py --copy-calibration",
+}
+
+
+def _counter_rows(pmc_dir, name, counter):
+    """Per dispatch (in dispatch order): (kernel name, summed counter value) from the rocprofv3
+    counter-collection CSV of run `name`."""
+    import csv
+    files = sorted(glob.glob(os.path.join(pmc_dir, "**", "%s*counter_collection.csv" % name), recursive=True))
+    if not files:
+        raise FileNotFoundError("no counter_collection.csv for run %r under %s" % (name, pmc_dir))
+    per = {}
+    for fn in files:
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (fn, int(r["Dispatch_Id"]))
+                k, v = per.get(key, (r["Kernel_Name"], 0.0))
+                per[key] = (k, v + float(r["Counter_Value"]))
+    return [per[k] for k in sorted(per)]
+
+
+def summarize_traffic(pmc_dir, out_path):
+    """HBM bytes per launch of the path and reduction kernels from the FETCH_SIZE / WRITE_SIZE runs
+    of TRAFFIC_RUNS, scaled by what the same counters read for a device copy of known size, stamped
+    with the hash of the sources they were measured on."""
+    copy_bytes = 1 << 30
+
+    def per_copy(name, counter):
+        rows = [v for k, v in _counter_rows(pmc_dir, name, counter) if v > 0]
+        return float(np.median(rows)) if rows else None
+
+    cf, cw = per_copy("cfetch", "FETCH_SIZE"), per_copy("cwrite", "WRITE_SIZE")
+    ff = copy_bytes / (cf * 1024.0)
+    wf = copy_bytes / (cw * 1024.0)
+    out = {"workload": "cfg2 (8x8 x 32 knots, 1e5 paths/knot, h=1e-4, 512^2 LUTs), FP64 paths, one assembly",
+           "source_hash": kernel_source_hash(), "fetch_factor": ff, "write_factor": wf,
+           "calibration": "1 GiB hipMemcpy device-to-device (bench.py --copy-calibration): FETCH_SIZE %.1f KiB, "
+                          "WRITE_SIZE %.1f KiB per copy" % (cf, cw),
+           "commands": list(TRAFFIC_RUNS.values()) + ["python3 bench.py --summarize-traffic gpurun_out/pmc"]}
+    fetch = _counter_rows(pmc_dir, "fetch", "FETCH_SIZE")
+    write = _counter_rows(pmc_dir, "write", "WRITE_SIZE")
+    for label, pat in (("path_kernel", "path_kernel"), ("reduce_kernel", "reduce_kernel")):
+        fk = [v for k, v in fetch if pat in k]
+        wk = [v for k, v in write if pat in k]
+        if not fk or not wk:
+            continue
+        rec = {"kernel": [k for k, v in fetch if pat in k][0][:120], "launches": len(fk),
+               "fetch_size_kib": float(np.mean(fk)), "write_size_kib": float(np.mean(wk)),
+               "fetch_bytes": float(np.mean(fk)) * 1024.0 * ff, "write_bytes": float(np.mean(wk)) * 1024.0 * wf}
+        rec["bytes_per_launch"] = rec["fetch_bytes"] + rec["write_bytes"]
+        out[label] = rec
+    if "path_kernel" in out:
+        out["bytes_per_launch"] = out["path_kernel"]["bytes_per_launch"]
+    out["algorithmic_bytes_per_launch"] = 36 * 3409 * 100000
+    out["algorithmic_bytes_note"] = ("exit records: a 32-byte FP64 record (x, y, Y, score) and a 4-byte metadata "
+                                     "word per path, 3.409e8 paths")
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def request_stream(duration_s, rate=200.0, seed=23, solve_paths=1000):
@@ -287,9 +406,14 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_cfg2.json"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device-to-device copies of 1 GiB (the reference kernel for HBM counters)")
+    ap.add_argument("--summarize-traffic", metavar="PMC_DIR",
+                    help="only write --traffic-file from the counter runs of TRAFFIC_RUNS under PMC_DIR")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.summarize_traffic:
+        summarize_traffic(args.summarize_traffic, args.traffic_file)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -335,10 +459,19 @@ def main():
     flops = FLOPS_PER_STEP.get(args.config, 302.0) * c["path_steps"]
     achieved = flops / avg_kernel_s / 1e12
     peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
-    traffic = None
+    # HBM bytes per launch, only if measured on these very sources (the counter runs cannot share
+    # a process with the timing); null otherwise
+    traffic, traffic_src = None, "none measured for this build"
     if args.precision == "fp64" and os.path.exists(args.traffic_file):
         try:
-            traffic = json.load(open(args.traffic_file)).get("bytes_per_launch")
+            tj = json.load(open(args.traffic_file))
+            if tj.get("source_hash") == kernel_source_hash():
+                traffic = tj.get("bytes_per_launch")
+                traffic_src = "%s (measured on source hash %s)" % (os.path.relpath(args.traffic_file, ROOT),
+                                                                   tj["source_hash"])
+            else:
+                traffic_src = "%s is for source hash %s, not this build's %s" % (
+                    os.path.relpath(args.traffic_file, ROOT), tj.get("source_hash"), kernel_source_hash())
         except (OSError, ValueError):
             traffic = None
     result = {
@@ -366,15 +499,18 @@ def main():
         "launch": {"grid_wgs": c["grid_wgs"], "wg_size": c["wg_size"], "item_paths": c["item_paths"],
                    "sub_batch": c["sub_batch"], "work_items": c["work_items"]},
         "device": info,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "kernel": ("path_kernel_tiled<%s> (LDS-tiled LUT)" % ("double" if args.precision == "fp64" else "float")
                                 if c["wg_size"] != 256 else "path_kernel (persistent, LUT from global memory)"),
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
-                     "note": ("no dense contraction: peak = MI355X %s; achieved counts %g algorithmic flops per "
-                              "path-step; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE (copy-calibrated): "
-                              "exit records stored when each path stops, and fifth-field LUT quads missing L2"
-                              % ("FP64 vector (= FP64 matrix) 78.6 TFLOP/s" if args.precision == "fp64"
+                     "traffic_source": traffic_src,
+                     "source_hash": kernel_source_hash(),
+                     "note": ("no MFMA (no step is a dense contraction): the kernel is bound by VALU issue; peak = "
+                              "MI355X %s; achieved counts %g algorithmic flops per path-step (transcendentals and "
+                              "Philox integer work not counted); traffic = HBM bytes per launch from "
+                              "FETCH_SIZE/WRITE_SIZE, copy-calibrated"
+                              % ("FP64 vector 78.6 TFLOP/s" if args.precision == "fp64"
                                  else "FP32 vector 157.3 TFLOP/s (packed; this build issues unpacked FP32)",
                                  FLOPS_PER_STEP.get(args.config, 302.0)))},
     }

@@ -301,7 +301,10 @@ typedef struct {
     pdds_problem problem;   /* PDE of every request (the Helmholtz k2 comes with each request) */
     pdds_run run;           /* h, seed, precision and launch parameters; stream is per request */
     double domain[4];       /* x0, y0, x1, y1 of every request's problem */
-    int64_t hbm_budget;     /* device bytes for admitted requests' tables and records (0: 16 GiB) */
+    int64_t hbm_budget;     /* device bytes for admitted requests (0: 16 GiB): each is counted by its
+                               stencil tables, the exit records of at most one round of its paths,
+                               its reduction partials and descriptors, and its G values -- an
+                               estimate of what it adds to the engine's buffers */
     int32_t max_inflight;   /* requests admitted at once (0: 256) */
     int32_t prep_threads;   /* host threads preparing requests (0: 4) */
     int64_t round_paths;    /* paths per round (0: 2^25) */
@@ -340,7 +343,7 @@ pdds_status pdds_server_submit(pdds_server *s, const pdds_request *r, uint64_t *
 /* PDDS_ERR_NOT_READY while the request runs; once done fills *res (and u for a solve) and
    releases the ticket */
 pdds_status pdds_server_poll(pdds_server *s, uint64_t ticket, pdds_result *res, double *u, int64_t cap);
-/* as poll, waiting for the request */
+/* as poll, waiting for the request; PDDS_ERR_INVALID if another thread took the result meanwhile */
 pdds_status pdds_server_wait(pdds_server *s, uint64_t ticket, pdds_result *res, double *u, int64_t cap);
 /* totals since creation: requests submitted and done, launch rounds, path-steps, device busy ms */
 pdds_status pdds_server_stats(pdds_server *s, int64_t *submitted, int64_t *done, int64_t *rounds,

@@ -47,7 +47,10 @@ hipError_t launch_path_f64(int pde, int geom, bool persistent, bool mixed, int g
     }
 }
 
-int path_tiled_wg(int precision) { return precision == PDDS_FP64 ? tiled_wg<double>() : tiled_wg<float>(); }
+int path_tiled_wg(int precision, int step) {
+    if (step == STEP_CV) return precision == PDDS_FP64 ? tiled_wg<double, STEP_CV>() : tiled_wg<float, STEP_CV>();
+    return precision == PDDS_FP64 ? tiled_wg<double, STEP_ANY>() : tiled_wg<float, STEP_ANY>();
+}
 
 template <bool MIXED, int GEOM, int FAM>
 static hipError_t path_tiled_f64_f(int grid, size_t lds, hipStream_t s, const PathKernelArgs &a) {
@@ -56,7 +59,7 @@ static hipError_t path_tiled_f64_f(int grid, size_t lds, hipStream_t s, const Pa
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<double>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<double, FAM>()), lds, s, a);
     return hipGetLastError();
 }
 

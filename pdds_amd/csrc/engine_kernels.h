@@ -466,7 +466,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
 // the next patch with paths left.  The 16 taps of a step are then LDS reads; only the
 // x-quads of field 4 (4 per step) and taps outside the tile go to the vector L1.
 // ----------------------------------------------------------------------------
-template <class R> constexpr int tiled_wg() { return 64 * 4 * path_waves<R, PDE_VARCOEF_LUT>(); }
+// one work-group per CU holding every wave the register budget allows (the run-time step-family
+// variants one wave less, as in path_kernel)
+template <class R, int FAM> constexpr int tiled_wg() { return 64 * 4 * path_waves_f<R, PDE_VARCOEF_LUT, FAM>(); }
 
 __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) {
     int lo = 0, hi = n;
@@ -479,7 +481,7 @@ __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) 
 }
 
 template <class R, bool MIXED, int GEOM, int FAM>
-__global__ void __launch_bounds__(tiled_wg<R>()) __attribute__((amdgpu_waves_per_eu(path_waves<R, PDE_VARCOEF_LUT>())))
+__global__ void __launch_bounds__((tiled_wg<R, FAM>())) __attribute__((amdgpu_waves_per_eu(path_waves_f<R, PDE_VARCOEF_LUT, FAM>())))
 path_kernel_tiled(PathKernelArgs a) {
     extern __shared__ float4 tile[];
     __shared__ int s_patch;

@@ -28,7 +28,11 @@ namespace {
 
 enum ReqState { RS_PREP = 0, RS_READY, RS_ADMITTED, RS_DONE };
 
-constexpr int64_t kRecordBytes = 4 * 8 + 1 + 4;   // exit record bytes per path
+// device bytes per path of a round: its exit record and metadata word
+constexpr int64_t kRecordBytes = (int64_t)sizeof(ExitRec) + (int64_t)sizeof(int32_t);
+// device bytes per work item of a knot besides its records and partials: the item's descriptor
+// and statistics, and (counted per item, an over-estimate) the knot's descriptor and statistics
+constexpr int64_t kKnotBytes = (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) + sizeof(ItemDesc) + sizeof(ItemStat));
 
 struct SrvReq {
     uint64_t ticket = 0;
@@ -50,6 +54,13 @@ struct SrvReq {
     std::vector<double> G, B;
     std::vector<double> u;
 };
+
+// work items of a knot of n paths in an assembly with run r (as run_assembly cuts them)
+int64_t items_of(const pdds_run &r, int64_t n) {
+    const int S = r.sub_batch > 0 ? r.sub_batch : 32;
+    const int64_t P = r.item_paths > 0 ? r.item_paths : choose_P(n, S);
+    return (n + P - 1) / P;
+}
 
 double table_bytes(const SrvReq &r) {
     return (double)r.sten.size() * sizeof(StencilDesc) + 8.0 * (r.entx.size() + r.enty.size() + r.ipsi.size()) +
@@ -108,9 +119,14 @@ void prepare(pdds_server *s, SrvReq *r) {
                 else r->mc.push_back(q);
             }
             r->res.n_knots = n;
+            // tables, the records of at most one round of its paths, per knot descriptors and
+            // statistics, and its CSR values: G and its variance (16 B per entry), and the
+            // reduction partials (2 per entry and work item)
             const int64_t paths = (int64_t)r->mc.size() * r->rq.n_paths;
+            const int64_t items_per_knot = items_of(s->cfg.run, r->rq.n_paths);
             r->footprint = (int64_t)table_bytes(*r) + std::min<int64_t>(paths, s->cfg.round_paths) * kRecordBytes +
-                           16 * (int64_t)M.row_ptr.back();
+                           (int64_t)r->mc.size() * items_per_knot * kKnotBytes +
+                           (16 + 16 * items_per_knot) * (int64_t)M.row_ptr.back();
         }
     }
     std::lock_guard<std::mutex> lk(s->mu);
@@ -308,6 +324,8 @@ pdds_status take_result(pdds_server *s, std::map<uint64_t, std::unique_ptr<SrvRe
     if (u && cap > 0)
         for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)r->u.size()); ++i) u[i] = r->u[i];
     s->reqs.erase(it);
+    // other threads waiting on this ticket wake up, find it gone and return PDDS_ERR_INVALID
+    s->cv_done.notify_all();
     return PDDS_OK;
 }
 
@@ -371,7 +389,8 @@ pdds_status pdds_server_submit(pdds_server *s, const pdds_request *rq, uint64_t 
         if (rq->n_sub < 2 || rq->n_node < 3) return PDDS_ERR_INVALID;
     } else {
         if (!(rq->x >= d[0] && rq->x <= d[2] && rq->y >= d[1] && rq->y <= d[3])) return PDDS_ERR_INVALID;
-        r->footprint = std::min<int64_t>(rq->n_paths, s->cfg.round_paths) * kRecordBytes;
+        r->footprint = std::min<int64_t>(rq->n_paths, s->cfg.round_paths) * kRecordBytes +
+                       items_of(s->cfg.run, rq->n_paths) * kKnotBytes;
         if (r->footprint > s->cfg.hbm_budget) return PDDS_ERR_FULL;
     }
     SrvReq *rp = r.get();
@@ -414,11 +433,15 @@ pdds_status pdds_server_poll(pdds_server *s, uint64_t ticket, pdds_result *res, 
 pdds_status pdds_server_wait(pdds_server *s, uint64_t ticket, pdds_result *res, double *u, int64_t cap) {
     if (!s) return PDDS_ERR_INVALID;
     std::unique_lock<std::mutex> lk(s->mu);
+    // the ticket is looked up afresh on every wake-up: another thread may have taken the result
+    // (poll or wait on the same ticket) and released the request meanwhile
+    s->cv_done.wait(lk, [&] {
+        auto f = s->reqs.find(ticket);
+        return f == s->reqs.end() || f->second->state == RS_DONE;
+    });
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return PDDS_ERR_INVALID;
-    SrvReq *r = it->second.get();
-    s->cv_done.wait(lk, [&] { return r->state == RS_DONE; });
-    return take_result(s, s->reqs.find(ticket), res, u, cap);
+    return take_result(s, it, res, u, cap);
 }
 
 pdds_status pdds_server_stats(pdds_server *s, int64_t *submitted, int64_t *done, int64_t *rounds, int64_t *path_steps,

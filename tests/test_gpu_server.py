@@ -122,3 +122,28 @@ def test_budget_and_timing_fields(eng):
     srv.close()
     assert r["status"] == 0 and 0.0 <= r["queue_ms"] <= r["latency_ms"] and r["path_steps"] > 0
     assert b["path_steps"] == 0 and b["u_var"] == 0.0
+
+
+def test_two_waiters_on_one_ticket(eng):
+    """Two threads wait on the same ticket: one gets the result, the other PDDS_ERR_INVALID once
+    the result is taken (the ticket is looked up again after every wake-up)."""
+    import threading
+    from pdds_amd._capi import ERR_INVALID, PddsError
+    srv = _server(eng)
+    t = srv.submit_point(0.25, -0.1, 200000, k2=1.0, stream=61)
+    got, errs = [], []
+
+    def waiter():
+        try:
+            got.append(srv.wait(t))
+        except PddsError as e:
+            errs.append(e.status)
+
+    th = [threading.Thread(target=waiter) for _ in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(60)
+    srv.close()
+    assert not any(x.is_alive() for x in th)
+    assert len(got) == 1 and got[0]["status"] == 0 and errs == [ERR_INVALID]

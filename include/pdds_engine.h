@@ -278,6 +278,23 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
                             int32_t n, const double *x, const double *y, const int64_t *n_paths,
                             double *u, double *u_var, pdds_counters *ctr);
 
+/* ---- point evaluation that allocates no device memory ----
+   No counterpart in the reference.  For callers that cannot allocate device memory: code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  Arguments, results
+   and errors are those of pdds_point_eval (bit-identical results), but every device buffer the
+   call needs (exit records, work items, reduction partials, statistics, queue counters and the
+   rectangle's stencil table) is taken from one fixed array of PDDS_NOALLOC_SCRATCH_BYTES that
+   each device holds from the library's load until the process ends; the call makes no hipMalloc
+   or hipFree.  It returns PDDS_ERR_FULL, before any device work, when its buffers would exceed
+   that array: about 36 bytes per path (some 1.8e6 paths per call in total).  Every call on one
+   device shares the array, which keeps the last call's contents (what must start at zero is
+   cleared by each call); overlapping calls on one device would race, so the library serialises
+   them with a per-device lock (they wait for each other, also across engines). */
+#define PDDS_NOALLOC_SCRATCH_BYTES ((int64_t)64 << 20)
+pdds_status pdds_point_eval_noalloc(pdds_engine *e, const pdds_problem *p, const pdds_run *r,
+                                    const double glob[4], int32_t n, const double *x, const double *y,
+                                    const int64_t *n_paths, double *u, double *u_var, pdds_counters *ctr);
+
 /* ---- (I + G) u = B by BiCGSTAB on the device, zero initial guess ----
    rows given without the identity (the mesh CSR pattern). Returns iterations in *iters. */
 pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_ptr, const int32_t *cols,

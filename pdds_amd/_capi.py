@@ -137,6 +137,8 @@ EXPORTS = {
                                   C.POINTER(Run), C.POINTER(TuneReport)]),
     "pdds_point_eval": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _d, C.c_int32, _d, _d, _i64,
                                     _d, _d, C.POINTER(Counters)]),
+    "pdds_point_eval_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _d, C.c_int32, _d, _d,
+                                            _i64, _d, _d, C.POINTER(Counters)]),
     "pdds_solve": (C.c_int32, [C.c_void_p, C.c_int32, _i64, _i32, _d, _d, _d, C.c_double, C.c_int32, _i32, _d]),
     "pdds_server_create": (C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "pdds_server_destroy": (None, [C.c_void_p]),

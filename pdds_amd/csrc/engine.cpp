@@ -31,9 +31,51 @@ pdds_status fail(pdds_engine *e, pdds_status s, const std::string &msg) {
     return s;
 }
 
-// grow-only device buffer for slot i
+// ---------------------------------------------------------------------------
+// No-allocation scratch: one fixed device array per device, held from the library's load until
+// the process ends.  In host code `scratch` names a host-side stand-in, not device memory; its
+// device address on each device comes from hipGetSymbolAddress (noalloc_scratch_of), and only
+// that address is ever handed to kernels or copies.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PDDS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_call_mu[kMaxDevices];   // serialises the calls that use a device's array
+}  // namespace
+
+// Device address of `device`'s scratch array, looked up once per device with that device
+// current (the caller has made it current).
+static hipError_t noalloc_scratch_of(int device, char **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess) return r;
+        g_scratch_ptr[device] = (char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// device buffer for slot i: grow-only and kept by the engine, or, while the engine has an arena
+// set, the next piece of the arena
 pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
     if (bytes == 0) bytes = 8;
+    if (e->arena) {
+        Arena &a = *e->arena;
+        const size_t off = (a.used + 255) & ~(size_t)255;
+        if (off > a.cap || bytes > a.cap - off)
+            return fail(e, PDDS_ERR_FULL, "the call needs more device memory than the " +
+                                              std::to_string(a.cap >> 20) + " MiB no-allocation scratch");
+        *out = a.base + off;
+        a.used = off + bytes;
+        return PDDS_OK;
+    }
     DevBuf &b = e->bufs[slot];
     if (b.n < bytes) {
         if (b.p) (void)hipFree(b.p);
@@ -293,7 +335,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         PatchDesc *dpatch = nullptr;
         if (tiled && (st = upload(e, B_PATCH, patches, &dpatch))) return st;
         if ((st = dbuf(e, B_COUNTER, nctr * sizeof(unsigned long long), &p))) return st; dctr = (unsigned long long *)p;
-        HIPCHK(hipMemsetAsync(dctr, 0, nctr * sizeof(unsigned long long), e->stream));
+        HIPCHK(launch_fill_u64(dctr, (int64_t)nctr, 0ULL, e->stream));
         // path kernel
         PathKernelArgs pa;
         std::memset(&pa, 0, sizeof(pa));
@@ -517,6 +559,10 @@ pdds_status pdds_engine_create(int device, pdds_engine **out) {
         e->lds_max = std::min<size_t>(e->lds_max, (size_t)160 * 1024);
     }
     if (e->n_cu <= 0) e->n_cu = 256;
+    // this device's no-allocation scratch array (looked up once per device); without it only
+    // pdds_point_eval_noalloc is unavailable
+    char *sp = nullptr;
+    if (noalloc_scratch_of(device, &sp) == hipSuccess) e->scratch = sp;
     *out = e;
     return PDDS_OK;
 }
@@ -528,6 +574,7 @@ void pdds_engine_destroy(pdds_engine *e) {
     for (auto &b : e->bufs)
         if (b.p) (void)hipFree(b.p);
     for (auto &t : e->tables) t.release();
+    e->dom.release();
     for (auto &ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1136,33 +1183,25 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
     return PDDS_OK;
 }
 
-pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
-                            int32_t n, const double *x, const double *y, const int64_t *n_paths, double *u,
-                            double *u_var, pdds_counters *ctr) {
-    if (!e) return PDDS_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(e->mu);
-    double t0 = now_ms();
+// Feynman-Kac estimates of u at points (Solve_OMP_Analytic without a stencil,
+// Solvers/EMFKACSolver.hpp:686-691): paths from each point stop on the rectangle glob.  The
+// rectangle's one-stencil table is kept by the engine and rebuilt only when glob changes.
+// noalloc: every device buffer, that table included, is taken from the device's scratch array
+// and nothing is allocated; the caller holds the engine lock (and, with noalloc, the device's
+// scratch lock).
+static pdds_status point_eval_locked(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
+                                     int32_t n, const double *x, const double *y, const int64_t *n_paths, double *u,
+                                     double *u_var, pdds_counters *ctr, bool noalloc) {
+    const double t0 = now_ms();
     if (!glob || !x || !y || !n_paths || !u || n < 0) return fail(e, PDDS_ERR_INVALID, "null argument");
     pdds_status st = check_run(e, r);
     if (st) return st;
     if (!(glob[2] > glob[0]) || !(glob[3] > glob[1])) return fail(e, PDDS_ERR_INVALID, "bad rectangle");
     PdeParams pp;
     if ((st = make_params(e, p, r, pp))) return st;
-    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
-    StencilDesc d;
-    std::memset(&d, 0, sizeof(d));
-    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.bbox[i] = glob[i]; d.inv_c2[i] = 1.0; }
-    d.gmask = 15;
-    std::vector<StencilDesc> sten = {d};
-    std::vector<double> entx, enty, ipsi;
-    std::vector<int8_t> ents;
-    std::vector<int32_t> e1, e2;
-    DevTables T;
-    std::vector<double> entic2;
-    if ((st = build_tables(e, sten, entx, enty, ents, entic2, e1, e2, ipsi, T))) return st;
-    struct Rel { DevTables &t; ~Rel() { t.release(); } } rel{T};
     std::vector<AsmKnot> knots;
     std::vector<int> kmap;
+    int64_t paths = 0;
     for (int i = 0; i < n; ++i) {
         double ex, ey, nx, ny;
         int kd;
@@ -1176,16 +1215,89 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
         if (n_paths[i] <= 0) return fail(e, PDDS_ERR_INVALID, "n_paths must be positive");
         knots.push_back(AsmKnot{x[i], y[i], (uint32_t)i, 0, n_paths[i], 0});
         kmap.push_back(i);
+        paths += n_paths[i];
     }
+    if (noalloc) {
+        // exit records, and per point its descriptor, statistics and work items (at most 17 of
+        // them, P >= N / 16, each with its descriptor and statistics), with room for alignment
+        const int64_t need = paths * (int64_t)(sizeof(ExitRec) + sizeof(int32_t)) +
+                             (int64_t)knots.size() * (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) +
+                                                                17 * (sizeof(ItemDesc) + sizeof(ItemStat))) +
+                             ((int64_t)1 << 16);
+        if (need > (int64_t)PDDS_NOALLOC_SCRATCH_BYTES)
+            return fail(e, PDDS_ERR_FULL, std::to_string(paths) + " paths at " + std::to_string(knots.size()) +
+                                              " points need about " + std::to_string(need >> 20) +
+                                              " MiB of device buffers; the no-allocation scratch holds " +
+                                              std::to_string(PDDS_NOALLOC_SCRATCH_BYTES >> 20) + " MiB");
+    }
+    if (hipSetDevice(e->device) != hipSuccess) return fail(e, PDDS_ERR_DEVICE, "hipSetDevice");
+    StencilDesc d;
+    std::memset(&d, 0, sizeof(d));
+    for (int i = 0; i < 4; ++i) { d.rect[i] = glob[i]; d.bbox[i] = glob[i]; d.inv_c2[i] = 1.0; }
+    d.gmask = 15;
     AsmOut o;
-    st = run_assembly(e, p, r, glob, {&T}, knots, 0, nullptr, nullptr, o);
-    if (st) return st;
+    if (noalloc) {
+        Arena arena;
+        arena.base = e->scratch;
+        arena.cap = PDDS_NOALLOC_SCRATCH_BYTES;
+        struct Scope {
+            pdds_engine *e;
+            ~Scope() { e->arena = nullptr; }
+        } scope{e};
+        e->arena = &arena;
+        // the table lives in the scratch array: the stencil descriptor, and one zeroed piece
+        // standing for its (empty) entry arrays, which no kernel reads
+        void *ps, *pz;
+        if ((st = dbuf(e, B_STEN, sizeof(StencilDesc), &ps)) || (st = dbuf(e, B_ENTX, 256, &pz))) return st;
+        HIPCHK(hipMemcpyAsync(ps, &d, sizeof(d), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(launch_fill_u64((unsigned long long *)pz, 256 / 8, 0ULL, e->stream));
+        DevTables T;
+        T.sten = (StencilDesc *)ps;
+        T.entx = T.enty = T.ipsi = T.entic2 = (double *)pz;
+        T.ents = (int8_t *)pz;
+        T.e1 = T.e2 = (int32_t *)pz;
+        T.host_sten = {d};
+        T.n_sten = 1;
+        st = run_assembly(e, p, r, glob, {&T}, knots, 0, nullptr, nullptr, o);
+        if (st) return st;
+    } else {
+        DevTables &T = e->dom;
+        if (!T.sten || std::memcmp(e->dom_rect, glob, sizeof(e->dom_rect)) != 0) {
+            std::vector<StencilDesc> sten = {d};
+            std::vector<double> none;
+            std::vector<int8_t> none8;
+            std::vector<int32_t> none32;
+            if ((st = build_tables(e, sten, none, none, none8, none, none32, none32, none, T))) return st;
+            std::memcpy(e->dom_rect, glob, sizeof(e->dom_rect));
+        }
+        st = run_assembly(e, p, r, glob, {&T}, knots, 0, nullptr, nullptr, o);
+        if (st) return st;
+    }
     for (size_t i = 0; i < knots.size(); ++i) {
         u[kmap[i]] = o.kstats[i].B;
         if (u_var) u_var[kmap[i]] = o.kstats[i].BB - o.kstats[i].B * o.kstats[i].B;
     }
     fill_counters(ctr, o, now_ms() - t0);
     return PDDS_OK;
+}
+
+pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
+                            int32_t n, const double *x, const double *y, const int64_t *n_paths, double *u,
+                            double *u_var, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    return point_eval_locked(e, p, r, glob, n, x, y, n_paths, u, u_var, ctr, false);
+}
+
+pdds_status pdds_point_eval_noalloc(pdds_engine *e, const pdds_problem *p, const pdds_run *r, const double glob[4],
+                                    int32_t n, const double *x, const double *y, const int64_t *n_paths, double *u,
+                                    double *u_var, pdds_counters *ctr) {
+    if (!e) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!e->scratch) return fail(e, PDDS_ERR_DEVICE, "the no-allocation scratch array of this device is unavailable");
+    // every engine on this device shares its array: calls wait for each other instead of racing
+    std::lock_guard<std::mutex> lk2(g_scratch_call_mu[e->device]);
+    return point_eval_locked(e, p, r, glob, n, x, y, n_paths, u, u_var, ctr, true);
 }
 
 // ---------------------------------------------------------------- paths ----

@@ -398,6 +398,20 @@ hipError_t launch_finalize(int grid, int wg, hipStream_t s, const FinalizeArgs &
     return hipGetLastError();
 }
 
+// Sets n 64-bit words to v: the path queues' counters are cleared by a kernel rather than a
+// memset node, so the sequence stays capturable in a HIP graph.
+__global__ void fill_u64_kernel(unsigned long long *p, int64_t n, unsigned long long v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int wg = 256;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n + wg - 1) / wg)), dim3(wg), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 __global__ void lut_eval_kernel(LutView L, int field, int64_t n, const double *x, const double *y, double *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -41,6 +41,13 @@ struct DevTables {
     }
 };
 
+// A fixed region of device memory handed out front to back (256-byte aligned): while an engine
+// has one set, dbuf takes every buffer from it instead of allocating.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct pdds_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -52,9 +59,13 @@ struct pdds_engine {
     std::vector<DevBuf> bufs;   // grow-only scratch, indexed by slot
     DevTables tables[2];        // two most recent (mesh, quirk) table sets
     int tables_next = 0;
+    DevTables dom;              // one-stencil table of the rectangle point evaluations stop on
+    double dom_rect[4] = {0, 0, 0, 0};
     int64_t record_budget = (int64_t)24 << 30;   // bytes of exit records per assembly batch
     int path_wgs_per_cu = 8;                      // persistent path-kernel work-groups per CU
     size_t lds_max = 64 * 1024;                   // LDS bytes a work-group may use
+    Arena *arena = nullptr;     // set only inside pdds_point_eval_noalloc
+    char *scratch = nullptr;    // device address of this device's no-allocation scratch array
 };
 
 struct pdds_lut {

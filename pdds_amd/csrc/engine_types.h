@@ -189,5 +189,6 @@ hipError_t launch_paths_f32(int pde, int grid, int wg, hipStream_t s, const Path
 hipError_t launch_finalize(int grid, int wg, hipStream_t s, const FinalizeArgs &f);
 hipError_t launch_lut_eval(const LutView &L, int field, int64_t n, const double *x, const double *y, double *out,
                            hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t s);
 
 }  // namespace pdds

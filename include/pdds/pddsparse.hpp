@@ -41,8 +41,11 @@ struct bvp {
     int32_t pde = PDDS_PDE_MONEGROS;
     double k2 = 0.0;
     const pdds_lut *lut = nullptr;
+    int32_t neumann = 0;     // reflecting global sides (PDDS_SIDE_* bits), the reference's bvp.Neumann
+    double varphi = 0.0;     // Robin coefficient on those sides (<= 0)
+    double rho = 0.0;        // Lepingle step within this distance of a reflecting side (0: always)
     std::function<double(double, double)> u;   // analytic solution (solution.csv); optional
-    pdds_problem desc() const { pdds_problem p{pde, k2, lut}; return p; }
+    pdds_problem desc() const { pdds_problem p{pde, k2, lut, neumann, varphi, rho}; return p; }
 };
 
 // Owns one engine (one device).

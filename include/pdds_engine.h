@@ -62,10 +62,27 @@ pdds_status pdds_engine_info(const pdds_engine *e, int32_t *n_cu, char *name, in
 #define PDDS_PDE_VARCOEF 2       /* div form 1/2 div(s s^T grad u) + b.grad u - c u = f, u = e^x cos y */
 #define PDDS_PDE_VARCOEF_LUT 3   /* PDE 2 with sigma11, sigma22, b1, b2, c_ref from a 5-field LUT */
 #define PDDS_PDE_HELMHOLTZ 4     /* u as PDE 1, c = -k2 */
+/* Boundary sides of the global rectangle, the bits of pdds_problem.refl */
+#define PDDS_SIDE_N 1
+#define PDDS_SIDE_S 2
+#define PDDS_SIDE_E 4
+#define PDDS_SIDE_W 8
 typedef struct {
     int32_t pde;
     double k2;              /* PDDS_PDE_HELMHOLTZ only */
     const pdds_lut *lut;    /* PDDS_PDE_VARCOEF_LUT only */
+    /* Reflecting (Neumann / Robin) sides, the reference's bvp.Neumann (BVPs/BVP.hpp:133) with
+       psi and varphi (:125-130): paths reflect off the global sides in refl (PDDS_SIDE_* bits;
+       0: every side Dirichlet; one side at least must absorb, so refl != 15) by the Lepingle step
+       (Integrators/FKACIntegrator.hpp:114-165) and stop only on the others.  There du/dn = varphi u + psi with varphi = robin (<= 0; 0 for
+       Neumann) and psi taken from the problem's solution (psi = grad u . n - varphi u).  rho: the
+       Lepingle step is taken within this distance of a reflecting side (the reference's rho),
+       0: on every step whose nearest side reflects.  pdds_paths reads refl over its rect's own
+       sides.  Rectangular domains only; knots on a reflecting side are not supported in an
+       assembly. */
+    int32_t refl;
+    double robin;
+    double rho;
 } pdds_problem;
 
 /* ---- run parameters ---- */

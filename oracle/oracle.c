@@ -288,21 +288,24 @@ static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
    Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29):
    xi += Y v . dW.  VR step (:31-44, grad holds grad u / u so v = mu): Y += c Y h + Y mu . dW
    and X's drift is b - sigma mu. */
-static inline void orc_step(int step, orc_state *s, double h, double sqrth, double n1, double n2) {
+static inline void orc_step(int step, orc_state *s, double h, double sqrth, double n1, double n2, double lz,
+                            double ly) {
+    /* lz = psi Y L, ly = varphi Y L: the local-time terms of a reflecting side (0 elsewhere),
+       Z += f Y h + psi Y L and Y += c Y h + varphi Y L (Integrators/FKACIntegrator.hpp:156-158) */
     double i0 = sqrth * n1, i1 = sqrth * n2;
     double v0 = -(s->sig[0] * s->grad[0] + s->sig[2] * s->grad[1]);
     double v1 = -(s->sig[1] * s->grad[0] + s->sig[3] * s->grad[1]);
     double dxi = s->Y * (v0 * i0 + v1 * i1);
     if (step != 2) s->xi = s->xi + dxi;
-    s->Z = s->Z + s->f * s->Y * h;
+    s->Z = s->Z + (s->f * s->Y * h + lz);
     double Yold = s->Y;
     double b0 = s->b[0], b1 = s->b[1];
     if (step == 2) {
-        s->Y = Yold + (s->c * Yold * h + dxi);
+        s->Y = Yold + ((s->c * Yold * h + dxi) + ly);
         b0 = b0 - (s->sig[0] * v0 + s->sig[1] * v1);
         b1 = b1 - (s->sig[2] * v0 + s->sig[3] * v1);
     } else {
-        s->Y = Yold + s->c * Yold * h;
+        s->Y = Yold + (s->c * Yold * h + ly);
     }
     double nx = s->x + (b0 * h + (s->sig[0] * i0 + s->sig[1] * i1));
     double ny = s->y + (b1 * h + (s->sig[2] * i0 + s->sig[3] * i1));
@@ -323,12 +326,131 @@ static inline int orc_inside(const orc_problem *pb, orc_state *s, int geom, cons
     return d < beta * nrm * sqrth;
 }
 
+/* ---- reflecting sides: the Lepingle step (Integrators/FKACIntegrator.hpp:114-165) ---- */
+/* Nearest side of a rectangle named by its outward normal (orc_rect_dist's): 0 N, 1 S, 2 E, 3 W,
+   corners 4 NE, 5 NW, 6 SE, 7 SW. */
+static inline int orc_side_of_normal(double nx, double ny) {
+    if (nx == 0.0) return ny > 0.0 ? 0 : 1;
+    if (ny == 0.0) return nx > 0.0 ? 2 : 3;
+    return ny > 0.0 ? (nx > 0.0 ? 4 : 5) : (nx > 0.0 ? 6 : 7);
+}
+/* whether that side reflects (a corner: both of its sides) */
+static inline int orc_reflects(int kind, int mask) {
+    static const int sides[8][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 2}, {0, 3}, {1, 2}, {1, 3}};
+    return ((mask >> sides[kind][0]) & 1) && ((mask >> sides[kind][1]) & 1);
+}
+/* E ~ Exp(1) for step `step`: a Philox draw of its own, counter word 3 = 2^31 */
+static inline double orc_exp_draw(const orc_run *run, uint32_t knot, uint32_t path, int64_t step) {
+    uint32_t key[2] = {run->seed, run->stream};
+    uint32_t ctr[4] = {(uint32_t)step, path, knot, 0x80000000u};
+    uint32_t w[4];
+    orc_philox4x32_10(ctr, key, w);
+    return -orc_log(run->normals == 0 ? orc_u01_53(w[0], w[1]) : orc_u01_24(w[0]));
+}
+/* One step from X whose nearest side (outward normal n, signed distance d) reflects.  Lepingle
+   (lep): the maximum of the step's Brownian bridge in the normal direction is
+   M = (u + sqrt(u^2 + 2 |n^T sigma|^2 h E)) / 2, u the normal displacement of the free step;
+   the local time L = max(0, M + d) pushes the new X back along n.  The functionals take L and
+   the pending local time lt, with psi at the old X's projection onto the side (the reference
+   takes it at the old X itself, O(sqrt h) inside, which costs the estimate half an order). */
+static inline void orc_refl_step(const orc_problem *pb, orc_state *s, double h, double sqrth, double n1, double n2,
+                                 int lep, double E, double nx, double ny, double d, double lt) {
+    double L = 0.0;
+    if (lep) {
+        const double i0 = sqrth * n1, i1 = sqrth * n2;
+        const double ga = nx * s->sig[0] + ny * s->sig[2], gb = nx * s->sig[1] + ny * s->sig[3];
+        const double uc = (ga * i0 + gb * i1) + (nx * s->b[0] + ny * s->b[1]) * h;
+        const double M = 0.5 * (uc + sqrt((ga * ga + gb * gb) * ((2.0 * h) * E) + uc * uc));
+        L = fmax(0.0, M + d);
+    }
+    const double Lt = L + lt;
+    double lz = 0.0, ly = 0.0;
+    if (Lt > 0.0) {
+        /* boundary data at X's projection onto the side, X - d n */
+        double sig[4], b[2], c, f, grad[2], g, u;
+        orc_pde_eval(pb, s->x - d * nx, s->y - d * ny, sig, b, &c, &f, grad, &g, &u);
+        const double psi = (grad[0] * nx + grad[1] * ny) - pb->robin * g;
+        /* the local time weighed by (e^{varphi L} - 1) / (varphi L): Y's exact growth over it
+           (1 on a Neumann side; the reference's linear Y += varphi Y L costs half an order) */
+        const double x = pb->robin * Lt;
+        const double w = Lt * (x == 0.0 ? 1.0 : orc_expm1(x) / x);
+        lz = (psi * s->Y) * w;
+        ly = (pb->robin * s->Y) * w;
+    }
+    orc_step(pb->step, s, h, sqrth, n1, n2, lz, ly);
+    if (lep) {
+        s->x = s->x - L * nx;
+        s->y = s->y - L * ny;
+    }
+}
+
 static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double x0, double y0, int geom,
                               const double *rect, uint32_t knot, int64_t path0, int64_t npaths,
                               double *xt, double *yt, double *Y, double *Z, double *xi, int64_t *steps,
                               double *xt2, double *yt2, double *Y2, double *Z2, double *xi2, int64_t *steps2) {
     const double h = run->h, sqrth = sqrt(h);
     const int64_t cap = run->max_steps > 0 ? run->max_steps : INT64_MAX;
+    const int rmask = geom ? 0 : (pb->refl & 15);
+    if (rmask) {
+        /* reflecting sides (rectangles): the path carries its nearest side and the local time of
+           a projection back onto a reflecting side, and stops only on the absorbing ones */
+#ifdef _OPENMP
+        int nthr = run->threads > 0 ? run->threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthr)
+#endif
+        for (int64_t n = 0; n < npaths; ++n) {
+            uint32_t path = (uint32_t)(path0 + n);
+            orc_state s;
+            s.x = x0; s.y = y0; s.Y = 1.0; s.Z = 0.0; s.xi = 0.0;
+            orc_coeffs(pb, &s);
+            double ex, ey, rnx, rny, lt = 0.0;
+            double rd = orc_rect_dist(rect, s.x, s.y, &ex, &ey, &rnx, &rny);
+            int rk = orc_side_of_normal(rnx, rny);
+            int64_t st = 0, stp = 0;
+            for (int phase = 0; phase < (run->phase2 ? 2 : 1); ++phase) {
+                const double beta = phase == 0 ? run->beta : 0.0;
+                int64_t k = 0;
+                int go;
+                do {
+                    double n1, n2, nx, ny;
+                    orc_normal_pair(run, knot, path, st, &n1, &n2);
+                    const int lep = orc_reflects(rk, rmask) && (pb->rho <= 0.0 || rd > -pb->rho);
+                    const double E = lep ? orc_exp_draw(run, knot, path, st) : 0.0;
+                    orc_refl_step(pb, &s, h, sqrth, n1, n2, lep, E, rnx, rny, rd, lt);
+                    ++st; ++k;
+                    double d = orc_rect_dist(rect, s.x, s.y, &ex, &ey, &nx, &ny);
+                    int kind = orc_side_of_normal(nx, ny);
+                    int rnow = orc_reflects(kind, rmask);
+                    lt = 0.0;
+                    if (rnow && d > 0.0) {
+                        /* past a reflecting side: back onto it (Inside's reflect outcome,
+                           Integrators/FKACIntegrator.hpp:189-207) */
+                        lt = d;
+                        s.x = ex; s.y = ey;
+                        d = orc_rect_dist(rect, s.x, s.y, &ex, &ey, &nx, &ny);
+                        kind = orc_side_of_normal(nx, ny);
+                        rnow = orc_reflects(kind, rmask);
+                    }
+                    rd = d; rk = kind; rnx = nx; rny = ny;
+                    orc_coeffs(pb, &s);
+                    const double a = nx * s.sig[0] + ny * s.sig[2];
+                    const double b = nx * s.sig[1] + ny * s.sig[3];
+                    go = rnow || d < beta * sqrt(a * a + b * b) * sqrth;
+                } while (go && k < cap);
+                double *o[5] = {xt, yt, Y, Z, xi};
+                int64_t *os = steps;
+                if (phase == 1) { o[0] = xt2; o[1] = yt2; o[2] = Y2; o[3] = Z2; o[4] = xi2; os = steps2; }
+                if (o[0]) o[0][n] = ex;
+                if (o[1]) o[1][n] = ey;
+                if (o[2]) o[2][n] = s.Y;
+                if (o[3]) o[3][n] = s.Z;
+                if (o[4]) o[4][n] = s.xi;
+                if (os) os[n] = st - stp;
+                stp = st;
+            }
+        }
+        return;
+    }
 #ifdef _OPENMP
     int nthr = run->threads > 0 ? run->threads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthr)
@@ -344,7 +466,7 @@ static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double 
         do {
             double n1, n2;
             orc_normal_pair(run, knot, path, st, &n1, &n2);
-            orc_step(pb->step, &s, h, sqrth, n1, n2);
+            orc_step(pb->step, &s, h, sqrth, n1, n2, 0.0, 0.0);
             ++st;
             in = orc_inside(pb, &s, geom, rect, run->beta, sqrth, &ex, &ey);
         } while (in && st < cap);
@@ -360,7 +482,7 @@ static void orc_simulate_impl(const orc_problem *pb, const orc_run *run, double 
             do {
                 double n1, n2;
                 orc_normal_pair(run, knot, path, st + st2, &n1, &n2);
-                orc_step(pb->step, &s, h, sqrth, n1, n2);
+                orc_step(pb->step, &s, h, sqrth, n1, n2, 0.0, 0.0);
                 ++st2;
                 in = orc_inside(pb, &s, geom, rect, 0.0, sqrth, &ex, &ey);
             } while (in && st2 < cap);

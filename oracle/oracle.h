@@ -48,6 +48,11 @@ typedef struct {
     int step;               /* step family: 0 control variate (Integrators/FKACIntegrator.hpp:46-57),
                                1 plain, grad u taken as 0 (:20-29), 2 VR, the drift change by
                                mu = -sigma^T grad u / u (:31-44) */
+    /* reflecting sides of the stopping rectangle (bit 0 N, 1 S, 2 E, 3 W): the Lepingle step
+       (Integrators/FKACIntegrator.hpp:114-165) with du/dn = robin u + psi, psi from the
+       solution; rho: Lepingle step within this distance (0: always) */
+    int refl;
+    double robin, rho;
 } orc_problem;
 
 typedef struct {

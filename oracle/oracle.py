@@ -27,7 +27,8 @@ class Problem(C.Structure):
     _fields_ = [("pde", C.c_int), ("k2", C.c_double), ("lut", _f),
                 ("lut_nx", C.c_int), ("lut_ny", C.c_int), ("lut_nf", C.c_int),
                 ("lut_ox", C.c_double), ("lut_oy", C.c_double),
-                ("lut_sx", C.c_double), ("lut_sy", C.c_double), ("step", C.c_int)]
+                ("lut_sx", C.c_double), ("lut_sy", C.c_double), ("step", C.c_int),
+                ("refl", C.c_int), ("robin", C.c_double), ("rho", C.c_double)]
 
 
 class Run(C.Structure):
@@ -121,12 +122,14 @@ def sector_dist(g, x, y):
     return (d, o[0], o[1], o[2], o[3], int(k[0]))
 
 
-def problem(pde, k2=0.0, lut=None, step=0):
-    """step: 0 control variate, 1 plain, 2 VR (the engine's pdds_run.step values)."""
+def problem(pde, k2=0.0, lut=None, step=0, refl=0, robin=0.0, rho=0.0):
+    """step: 0 control variate, 1 plain, 2 VR (the engine's pdds_run.step values); refl: reflecting
+    sides of the stopping rectangle (bits N, S, E, W), robin: varphi there, rho: Lepingle band."""
     pb = Problem()
     pb.pde = pde
     pb.k2 = k2
     pb.step = step
+    pb.refl, pb.robin, pb.rho = refl, robin, rho
     if lut is not None:
         pb._lut_keep = lut["coef"]
         pb.lut = ptr(lut["coef"], _f)

@@ -21,7 +21,11 @@ _i64 = C.POINTER(C.c_int64)
 
 
 class Problem(C.Structure):
-    _fields_ = [("pde", C.c_int32), ("k2", C.c_double), ("lut", C.c_void_p)]
+    _fields_ = [("pde", C.c_int32), ("k2", C.c_double), ("lut", C.c_void_p), ("refl", C.c_int32),
+                ("robin", C.c_double), ("rho", C.c_double)]
+
+
+SIDE_N, SIDE_S, SIDE_E, SIDE_W = 1, 2, 4, 8
 
 
 class Run(C.Structure):

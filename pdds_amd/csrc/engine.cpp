@@ -95,6 +95,13 @@ pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r
     std::memset(&pp, 0, sizeof(pp));
     pp.k2 = p->k2;
     pp.step = r ? r->step : STEP_CV;
+    // at least one side absorbs, so that every stopping domain (a stencil rectangle, or the whole
+    // rectangle of a point evaluation) has a side where paths stop
+    if (p->refl < 0 || p->refl > 14 || !(p->robin <= 0.0) || !std::isfinite(p->robin) || !(p->rho >= 0.0))
+        return fail(e, PDDS_ERR_INVALID, "reflecting sides: refl in 0..14 (one side at least absorbs), robin <= 0, rho >= 0");
+    pp.refl = p->refl;
+    pp.robin = p->robin;
+    pp.rho = p->rho;
     if (p->pde == PDDS_PDE_VARCOEF_LUT) {
         if (!p->lut || p->lut->nf < 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
         pp.lut = p->lut->view;
@@ -207,9 +214,12 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     const int geom = tabs[0]->geom;
     for (const DevTables *t : tabs)
         if (t->geom != geom) return fail(e, PDDS_ERR_INVALID, "one assembly mixes stencil geometries");
+    if (pp.refl && geom != GeomRect) return fail(e, PDDS_ERR_UNSUPPORTED, "reflecting sides need rectangular stencils");
     // LDS-tiled form (coefficient LUT with a fifth-field quad plane, persistent launch): knots
-    // are grouped into patches of equal tile, tiles that fit in LDS first, in first-seen order
-    const bool tiled = persistent && prob->pde == PDDS_PDE_VARCOEF_LUT && r->lut_lds != 0 && pp.lut.nf == 5;
+    // are grouped into patches of equal tile, tiles that fit in LDS first, in first-seen order;
+    // not with reflecting sides, whose steps run in the persistent and plain kernels
+    const bool tiled =
+        persistent && prob->pde == PDDS_PDE_VARCOEF_LUT && r->lut_lds != 0 && pp.lut.nf == 5 && pp.refl == 0;
     const size_t tile_cap = e->lds_max > 1024 ? e->lds_max - 1024 : 0;
     std::vector<size_t> ord(knots.size());
     for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
@@ -292,6 +302,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         // knots of other streams or k2 than the run's (a serving round): per-lane values
         bool mixed = false;
         for (int q = 0; q < nk; ++q) mixed = mixed || kd[q].stream != r->stream || kd[q].k2 != pp.k2;
+        if (mixed && pp.refl)
+            return fail(e, PDDS_ERR_UNSUPPORTED, "reflecting sides with per-knot streams or k2 in one launch");
         const size_t shmem = reduce_lds_bytes(ne_cap, nnz_cap, S);
         if (shmem > 160 * 1024 || S > 65535)
             return fail(e, PDDS_ERR_UNSUPPORTED, "stencil or sub-batch too large for the LDS reduction buffers");
@@ -847,6 +859,15 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
     for (int q = 0; q < M.n_knots(); ++q) {
         int64_t n = paths_per_knot ? paths_per_knot[q] : n_paths;
         if (n < 0) return fail(e, PDDS_ERR_INVALID, "negative path count");
+        if (M.on_boundary[q] && pp.refl) {
+            // a knot on a reflecting side has no Dirichlet value (it would need a stencil of its own)
+            const double tol = 1e-12 * (1.0 + std::fabs(M.glob[2] - M.glob[0]));
+            const bool on[4] = {std::fabs(M.y0[q] - M.glob[3]) < tol, std::fabs(M.y0[q] - M.glob[1]) < tol,
+                                std::fabs(M.x0[q] - M.glob[2]) < tol, std::fabs(M.x0[q] - M.glob[0]) < tol};
+            for (int s = 0; s < 4; ++s)
+                if (on[s] && ((pp.refl >> s) & 1))
+                    return fail(e, PDDS_ERR_UNSUPPORTED, "a knot lies on a reflecting side (knot " + std::to_string(q) + ")");
+        }
         if (M.on_boundary[q]) {
             B[q] = host_g(p->pde, M.x0[q], M.y0[q], pp);
             BB[q] = B[q] * B[q];

@@ -26,6 +26,15 @@ static hipError_t path_f32_g(bool persistent, bool mixed, int grid, int wg, hipS
 template <int PDE>
 static hipError_t path_f32(int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                           const PathKernelArgs &a) {
+    if (a.pp.refl) {
+        // reflecting sides: rectangles, one stream and k2 per launch, every step family
+        if (mixed || geom != GeomRect) return hipErrorInvalidValue;
+        if (!persistent)
+            hipLaunchKernelGGL((plain_kernel<float, PDE, GeomRect, STEP_ANY, true>), dim3(grid), dim3(wg), 0, s, a);
+        else
+            hipLaunchKernelGGL((path_kernel<float, PDE, false, GeomRect, STEP_ANY, true>), dim3(grid), dim3(wg), 0, s, a);
+        return hipGetLastError();
+    }
     return geom == GeomDisc ? path_f32_g<PDE, GeomDisc>(persistent, mixed, grid, wg, s, a)
                             : path_f32_g<PDE, GeomRect>(persistent, mixed, grid, wg, s, a);
 }
@@ -67,7 +76,9 @@ hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hip
 
 template <int PDE>
 static hipError_t paths_f32(int grid, int wg, hipStream_t s, const PathsArgs &a) {
-    if (a.pp.step == STEP_CV)
+    if (a.pp.refl)
+        hipLaunchKernelGGL((paths_kernel<float, PDE, STEP_ANY, true>), dim3(grid), dim3(wg), 0, s, a);
+    else if (a.pp.step == STEP_CV)
         hipLaunchKernelGGL((paths_kernel<float, PDE, STEP_CV>), dim3(grid), dim3(wg), 0, s, a);
     else
         hipLaunchKernelGGL((paths_kernel<float, PDE, STEP_ANY>), dim3(grid), dim3(wg), 0, s, a);

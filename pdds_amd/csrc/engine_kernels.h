@@ -99,13 +99,17 @@ template <class R> struct PathState {
 // Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29; the plain
 // step's g is 0): xi += Y v . dW with v = -sigma^T g.  VR step (FKACIntegrator.hpp:31-44; g is
 // grad u / u, so v is mu): Y += Y mu . dW and the drift of X loses sigma mu; xi is not used.
-template <class R, int FAM>
-__device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h, R sqrth, R n1, R n2, int fam) {
+// LT: the step carries local-time terms lz = psi Y L and ly = varphi Y L of a reflecting side
+// (Integrators/FKACIntegrator.hpp:156-158: Z += f Y h + psi Y L, Y += c Y h + varphi Y L).
+template <class R, int FAM, bool LT = false>
+__device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h, R sqrth, R n1, R n2, int fam,
+                                        R lz = (R)0, R ly = (R)0) {
     R i0 = sqrth * n1, i1 = sqrth * n2;
     R v0 = -(co.s00 * co.g0 + co.s10 * co.g1);
     R v1 = -(co.s01 * co.g0 + co.s11 * co.g1);
     R dxi = s.Y * (v0 * i0 + v1 * i1);
     R dz = co.f * s.Y * h;
+    if constexpr (LT) dz = dz + lz;
     const bool vr = FAM != STEP_CV && fam == STEP_VR;
     if constexpr (sizeof(R) == 8) {
         if (!vr) s.xi = s.xi + dxi;
@@ -118,11 +122,15 @@ __device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h,
     R Yold = s.Y;
     R b0 = co.b0, b1 = co.b1;
     if (vr) {
-        s.Y = Yold + (co.c * Yold * h + dxi);
+        R dy = co.c * Yold * h + dxi;
+        if constexpr (LT) dy = dy + ly;
+        s.Y = Yold + dy;
         b0 = b0 - (co.s00 * v0 + co.s01 * v1);
         b1 = b1 - (co.s10 * v0 + co.s11 * v1);
     } else {
-        s.Y = Yold + co.c * Yold * h;
+        R dy = co.c * Yold * h;
+        if constexpr (LT) dy = dy + ly;
+        s.Y = Yold + dy;
     }
     R nx = s.x + (b0 * h + (co.s00 * i0 + co.s01 * i1));
     R ny = s.y + (b1 * h + (co.s10 * i0 + co.s11 * i1));
@@ -163,6 +171,70 @@ __device__ __forceinline__ void draw_normals(PathState<R> &s, uint32_t seed, uin
             n1 = s.pn1;
             n2 = s.pn2;
         }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Reflecting sides (Neumann / Robin): the Lepingle step (Integrators/FKACIntegrator.hpp:114-165)
+// ----------------------------------------------------------------------------
+// Whether the nearest side `kind` of a path (rect_dist's kind) reflects: bit s of rmask marks a
+// reflecting side; a corner reflects only if both of its sides do.
+PDDS_HD bool refl_kind(int kind, int rmask) {
+    if (kind < 4) return ((rmask >> kind) & 1) != 0;
+    const int v = (kind == 4 || kind == 5) ? 0 : 1;     // N for NE/NW, S for SE/SW
+    const int hs = (kind == 4 || kind == 6) ? 2 : 3;    // E for NE/SE, W for NW/SW
+    return ((rmask >> v) & 1) && ((rmask >> hs) & 1);
+}
+
+// Exponential variate (mean 1) of step st for the bridge maximum: a Philox draw of its own
+// (counter word 3 with its top bit set, so it never meets a normal draw's counter).
+template <class R>
+__device__ __forceinline__ R draw_exponential(uint32_t st, uint32_t path, uint32_t seed, uint32_t stream,
+                                              uint32_t knot) {
+    if constexpr (sizeof(R) == 8) {
+        Philox4 w = philox4x32_10(st, path, knot, 0x80000000u, seed, stream);
+        return -dlog(u01_53(w.v[0], w.v[1]));
+    } else {
+        Philox4 w = philox4x32_10(st, path, knot, 0x80000000u, seed, stream);
+        const float u = __builtin_fmaf((float)(w.v[0] >> 8), 5.9604644775390625e-08f, 2.98023223876953125e-08f);
+        return __builtin_amdgcn_logf(u) * -0.6931471805599453f;
+    }
+}
+
+// One step of a path whose nearest side (outward normal (nx, ny), signed distance d <= 0)
+// reflects.  lep: the Lepingle step -- the local time L is how far the maximum of the Brownian
+// bridge of the step in the normal direction, M = (u + sqrt(u^2 + 2 |N^T sigma|^2 h E)) / 2 with
+// u the step's normal displacement and E ~ Exp(1), passes the side, L = max(0, M + d), and the
+// new X is pushed back by L along the normal; else a plain step (L = 0).  The functionals take
+// this step's L and the local time lt of a projection at the end of the previous step
+// (Integrators/FKACIntegrator.hpp:156-158), with psi at the old X's projection onto the side
+// (the reference takes it at the old X, O(sqrt h) inside, which costs half an order).
+template <class R, int PDE, int FAM>
+__device__ __forceinline__ void refl_step(PathState<R> &s, const Coef<R> &co, const PdeParams &pp, R h, R sqrth,
+                                          R n1, R n2, int fam, bool lep, R E, R nx, R ny, R d, R lt) {
+    R L = (R)0;
+    if (lep) {
+        const R i0 = sqrth * n1, i1 = sqrth * n2;
+        const R ga = nx * co.s00 + ny * co.s10, gb = nx * co.s01 + ny * co.s11;
+        const R uc = (ga * i0 + gb * i1) + (nx * co.b0 + ny * co.b1) * h;
+        const R M = (R)0.5 * (uc + RM<R>::sqrt((ga * ga + gb * gb) * (((R)2.0 * h) * E) + uc * uc));
+        L = RM<R>::fmax((R)0, M + d);
+    }
+    const R Lt = L + lt;
+    R lz = (R)0, ly = (R)0;
+    if (Lt > (R)0) {
+        // boundary data at X's projection onto the side, X - d n; the local time weighed by
+        // (e^{varphi L} - 1) / (varphi L), Y's exact growth over it (1 on a Neumann side)
+        const R phi = (R)pp.robin;
+        const R psi = pde_psi<PDE, R>(s.x - d * nx, s.y - d * ny, nx, ny, phi, pp);
+        const R w = Lt * RM<R>::exprel(phi * Lt);
+        lz = (psi * s.Y) * w;
+        ly = (phi * s.Y) * w;
+    }
+    em_step<R, FAM, true>(s, co, h, sqrth, n1, n2, fam, lz, ly);
+    if (lep) {
+        s.x = s.x - L * nx;
+        s.y = s.y - L * ny;
     }
 }
 
@@ -236,7 +308,10 @@ template <int GEOM> constexpr int domain_len() { return GEOM == GeomDisc ? 6 : 4
 // ----------------------------------------------------------------------------
 // MIXED: the knots of the launch differ in Philox stream or Helmholtz k2 (serving rounds), so
 // each lane carries its knot's; otherwise both are the launch's and stay in scalar registers.
-template <class R, int PDE, bool MIXED, int GEOM, int FAM>
+// REFL: some global sides reflect (rectangles only): the path carries its nearest side (distance,
+// kind, outward normal) and the local time of a projection, and steps with refl_step while that
+// side reflects; only absorbing sides stop it.
+template <class R, int PDE, bool MIXED, int GEOM, int FAM, bool REFL = false>
 struct Walker {
     PathState<R> s;
     Coef<R> co;            // coefficients at the current X
@@ -248,10 +323,15 @@ struct Walker {
     uint32_t kstream;      // Philox key word 1 of the path's knot
     R k2;                  // Helmholtz parameter of the path's knot
     bool active, fresh;
+    // REFL: reflecting sides of the stopping domain, nearest side at the current X, pending
+    // local time
+    int rmask, rk;
+    R rd, rnx, rny, lt;
 
     __device__ __forceinline__ void init() {
         pn1 = 0; pn2 = 0; kid = 0; kstream = 0; gmask = 0; g = 0; k2 = 0;
         active = false; fresh = false;
+        rmask = 0; rk = 0; rd = 0; rnx = 0; rny = 0; lt = 0;
     }
 
     // take path gg of the batch; k is its knot
@@ -265,6 +345,10 @@ struct Walker {
         if constexpr (MIXED) {
             kstream = kd.stream;
             k2 = (R)kd.k2;
+        }
+        if constexpr (REFL) {
+            rmask = sd.gmask & a.pp.refl;   // stencil sides on reflecting global sides
+            lt = (R)0;
         }
         g = (uint32_t)gg;
         s.path = (uint32_t)(gg - kd.path_off);
@@ -292,7 +376,16 @@ struct Walker {
         if constexpr (DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) == 0);
         {
             PathState<R> t = s;
-            em_step<R, FAM>(t, co, h, sqrth, pn1, pn2, a.pp.step);
+            if constexpr (REFL) {
+                // the nearest side at the current X reflects: the Lepingle step within rho of it
+                const bool rside = refl_kind(rk, rmask);
+                const bool lep = rside && (a.pp.rho <= 0.0 || rd > -(R)a.pp.rho);
+                R E = (R)0;
+                if (stepping && lep) E = draw_exponential<R>(s.st, s.path, a.seed, strm, kid);
+                refl_step<R, PDE, FAM>(t, co, a.pp, h, sqrth, pn1, pn2, a.pp.step, lep, E, rnx, rny, rd, lt);
+            } else {
+                em_step<R, FAM>(t, co, h, sqrth, pn1, pn2, a.pp.step);
+            }
             s.x = stepping ? t.x : s.x;
             s.y = stepping ? t.y : s.y;
             s.Y = stepping ? t.Y : s.Y;
@@ -306,13 +399,29 @@ struct Walker {
         }
         R ex, ey, nx, ny;
         int kind;
-        const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        bool rnow = false;
+        if constexpr (REFL) {
+            // past a reflecting side: back onto it, the distance becoming local time for the next
+            // step (Inside's reflect outcome, Integrators/FKACIntegrator.hpp:189-207)
+            rnow = refl_kind(kind, rmask);
+            lt = (R)0;
+            if (stepping && rnow && d > (R)0) {
+                lt = d;
+                s.x = ex;
+                s.y = ey;
+                d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+                rnow = refl_kind(kind, rmask);
+            }
+            rd = d; rk = kind; rnx = nx; rny = ny;
+        }
         pde_coef_t<PDE, R, TILED, FAM>(s.x, s.y, a.pp, T, kk2, co);
         if constexpr (!DRAW_FIRST) draw_normals<R>(s, a.seed, strm, kid, pn1, pn2, (it & 1) != 0);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-        const bool go = d < beta * nrm * sqrth;
+        // a reflecting nearest side never stops the path
+        const bool go = rnow || d < beta * nrm * sqrth;
         const bool capped = (int64_t)s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
             write_exit<R, PDE, GEOM>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
@@ -377,11 +486,11 @@ template <class R, int PDE, int FAM> constexpr int path_waves_f() {
     return FAM == STEP_CV || path_waves<R, PDE>() < 3 ? path_waves<R, PDE>() : path_waves<R, PDE>() - 1;
 }
 
-template <class R, int PDE, bool MIXED, int GEOM, int FAM>
+template <class R, int PDE, bool MIXED, int GEOM, int FAM, bool REFL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_waves_f<R, PDE, FAM>()))) path_kernel(PathKernelArgs a) {
     const int lane = __lane_id();
     const LutTile none{nullptr, 0, 0, 0, 0};
-    Walker<R, PDE, MIXED, GEOM, FAM> w;
+    Walker<R, PDE, MIXED, GEOM, FAM, REFL> w;
     w.init();
     bool done = false;
     // The wave starts on its XCD's queue, so the paths (and LUT texels) of one XCD's L2
@@ -422,7 +531,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
 // would spill there and keep the default
 template <class R, int FAM> constexpr int plain_waves() { return sizeof(R) == 8 && FAM == STEP_CV ? 3 : 1; }
 
-template <class R, int PDE, int GEOM, int FAM>
+template <class R, int PDE, int GEOM, int FAM, bool REFL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_waves<R, FAM>()))) plain_kernel(PathKernelArgs a) {
     const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gg >= a.total) return;
@@ -442,18 +551,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
     pde_coef_t<PDE, R, false, FAM>(s.x, s.y, a.pp, none, k2, co);
     R ex, ey;
     int kind;
+    // REFL: reflecting sides, the nearest side at the current X and the pending local time (as
+    // in Walker::step)
+    const int rmask = REFL ? (sd.gmask & a.pp.refl) : 0;
+    R rd = (R)0, rnx = (R)0, rny = (R)0, lt = (R)0;
+    int rk = 0;
+    if constexpr (REFL) rd = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, rnx, rny, rk);
     bool go, capped;
     do {
         R n1, n2, nx, ny;
         draw_normals<R>(s, a.seed, kd.stream, kd.knot_id, n1, n2, (s.st & 1) == 0);
-        em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
+        if constexpr (REFL) {
+            const bool lep = refl_kind(rk, rmask) && (a.pp.rho <= 0.0 || rd > -(R)a.pp.rho);
+            const R E = lep ? draw_exponential<R>(s.st, s.path, a.seed, kd.stream, kd.knot_id) : (R)0;
+            refl_step<R, PDE, FAM>(s, co, a.pp, h, sqrth, n1, n2, a.pp.step, lep, E, rnx, rny, rd, lt);
+        } else {
+            em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
+        }
         s.st += 1;
-        const R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        bool rnow = false;
+        if constexpr (REFL) {
+            rnow = refl_kind(kind, rmask);
+            lt = (R)0;
+            if (rnow && d > (R)0) {
+                lt = d;
+                s.x = ex;
+                s.y = ey;
+                d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+                rnow = refl_kind(kind, rmask);
+            }
+            rd = d; rk = kind; rnx = nx; rny = ny;
+        }
         pde_coef_t<PDE, R, false, FAM>(s.x, s.y, a.pp, none, k2, co);
         const R ga = nx * co.s00 + ny * co.s10;
         const R gb = nx * co.s01 + ny * co.s11;
         const R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-        go = d < beta * nrm * sqrth;
+        go = rnow || d < beta * nrm * sqrth;
         capped = (int64_t)s.st >= a.max_steps;
     } while (go && !capped);
     write_exit<R, PDE, GEOM>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind, capped && go);
@@ -550,7 +684,8 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);
 // ----------------------------------------------------------------------------
 // Per-path kernel (MCinCUDA shape): one path per lane, records written out.
 // ----------------------------------------------------------------------------
-template <class R, int PDE, int FAM>
+// REFL: the sides of a.rect in the mask a.pp.refl reflect (as in Walker::step).
+template <class R, int PDE, int FAM, bool REFL = false>
 __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -564,6 +699,10 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
     Coef<R> co;
     pde_coef<PDE, R, FAM>(s.x, s.y, a.pp, co);
     R ex = s.x, ey = s.y;
+    const int rmask = REFL ? (a.pp.refl & 15) : 0;
+    R rd = (R)0, rnx = (R)0, rny = (R)0, lt = (R)0;
+    int rk = 0;
+    if constexpr (REFL) rd = rect_dist<R>(rect, s.x, s.y, ex, ey, rnx, rny, rk);
     for (int phase = 0; phase < (a.phase2 ? 2 : 1); ++phase) {
         const R beta = phase == 0 ? (R)a.beta : (R)0;
         const uint32_t st0 = s.st;
@@ -572,14 +711,33 @@ __global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
             R n1, n2, nx, ny;
             int kind;
             draw_normals<R>(s, a.seed, a.stream, a.knot, n1, n2, (s.st & 1) == 0);
-            em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
+            if constexpr (REFL) {
+                const bool lep = refl_kind(rk, rmask) && (a.pp.rho <= 0.0 || rd > -(R)a.pp.rho);
+                const R E = lep ? draw_exponential<R>(s.st, s.path, a.seed, a.stream, a.knot) : (R)0;
+                refl_step<R, PDE, FAM>(s, co, a.pp, h, sqrth, n1, n2, a.pp.step, lep, E, rnx, rny, rd, lt);
+            } else {
+                em_step<R, FAM>(s, co, h, sqrth, n1, n2, a.pp.step);
+            }
             s.st += 1;
             R d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+            bool rnow = false;
+            if constexpr (REFL) {
+                rnow = refl_kind(kind, rmask);
+                lt = (R)0;
+                if (rnow && d > (R)0) {
+                    lt = d;
+                    s.x = ex;
+                    s.y = ey;
+                    d = rect_dist<R>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+                    rnow = refl_kind(kind, rmask);
+                }
+                rd = d; rk = kind; rnx = nx; rny = ny;
+            }
             pde_coef<PDE, R, FAM>(s.x, s.y, a.pp, co);
             R ga = nx * co.s00 + ny * co.s10;
             R gb = nx * co.s01 + ny * co.s11;
             R nrm = RM<R>::sqrt(ga * ga + gb * gb);
-            go = (d < beta * nrm * sqrth) && ((int64_t)(s.st - st0) < a.max_steps);
+            go = (rnow || d < beta * nrm * sqrth) && ((int64_t)(s.st - st0) < a.max_steps);
         }
         double *o[5];
         int64_t *os;

@@ -189,6 +189,9 @@ PDDS_HD double dexpm1(double x) {
     return dexp(x) - 1.0;
 }
 
+// (e^x - 1) / x, 1 at x = 0 (the local-time weight of a Robin side)
+PDDS_HD double dexprel(double x) { return x == 0.0 ? 1.0 : dexpm1(x) / x; }
+
 PDDS_HD double dlog(double x) {
     if (!(x > 0.0)) return (x == 0.0) ? -__builtin_huge_val() : __builtin_nan("");
     if (x == __builtin_huge_val()) return x;
@@ -281,6 +284,7 @@ template <> struct RM<double> {
     static PDDS_HD double sin(double x) { return dsin(x); }
     static PDDS_HD double cos(double x) { return dcos(x); }
     static PDDS_HD double tanh(double x) { return dtanh(x); }
+    static PDDS_HD double exprel(double x) { return dexprel(x); }
     static PDDS_HD double div(double a, double b) { return a / b; }
     static PDDS_HD double floor(double x) { return __builtin_floor(x); }
     static PDDS_HD double fmin(double a, double b) { return __builtin_fmin(a, b); }
@@ -307,6 +311,13 @@ template <> struct RM<float> {
     }
     static __device__ __forceinline__ float sin(float x) { float s, c; sincos(x, s, c); return s; }
     static __device__ __forceinline__ float cos(float x) { float s, c; sincos(x, s, c); return c; }
+    // (e^x - 1) / x: Taylor terms to x^4 for |x| < 1/16 (truncation below 2^-24), else from exp
+    static __device__ __forceinline__ float exprel(float x) {
+        if (__builtin_fabsf(x) < 0.0625f)
+            return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(__builtin_fmaf(x, 1.0f / 120.0f, 1.0f / 24.0f), x,
+                                                                1.0f / 6.0f), x, 0.5f), x, 1.0f);
+        return (exp(x) - 1.0f) * __builtin_amdgcn_rcpf(x);
+    }
     static __device__ __forceinline__ float tanh(float x) {
         float ax = __builtin_fabsf(x);
         float e = exp(-2.0f * ax);

@@ -26,6 +26,9 @@ struct PdeParams {
     double k2;             // HELMHOLTZ (launches without per-knot values)
     LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c
     int32_t step;          // StepFamily: PLAIN takes grad u as 0, VR hands out grad u / u
+    int32_t refl;          // bit s: global side s (0 N, 1 S, 2 E, 3 W) reflects (Neumann / Robin)
+    double robin;          // varphi on reflecting sides (<= 0); psi = grad u . N - varphi u
+    double rho;            // Lepingle step within this distance of a reflecting side
 };
 
 // Monegros Poisson (BVPs/EDPs/Monegros_Poisson_CPP.hpp:4-81): u, grad u, Laplacian
@@ -124,6 +127,33 @@ template <int PDE, class R, int FAM = STEP_CV>
 PDDS_HD void pde_coef(R x, R y, const PdeParams &P, Coef<R> &co) {
     const LutTile none{nullptr, 0, 0, 0, 0};
     pde_coef_t<PDE, R, false, FAM>(x, y, P, none, (R)P.k2, co);
+}
+
+// Boundary data of a reflecting side with outward normal (nx, ny), the reference's psi and varphi
+// (BVPs/BVP.hpp:125-130; Z += psi Y L and Y += varphi Y L per local-time increment L,
+// Integrators/FKACIntegrator.hpp:156-158): the Robin condition du/dn = varphi u + psi of the
+// manufactured solution, varphi the problem's constant (0: Neumann), so psi = grad u . n - varphi u.
+template <int PDE, class R> PDDS_HD R pde_g(R x, R y, const PdeParams &P);
+template <int PDE, class R>
+PDDS_HD R pde_psi(R x, R y, R nx, R ny, R varphi, const PdeParams &P) {
+    R gx, gy;
+    if constexpr (PDE == PDE_MONEGROS) {
+        R u, lap;
+        monegros<R>(x, y, u, gx, gy, lap);
+    } else if constexpr (PDE == PDE_SINSIN || PDE == PDE_HELMHOLTZ) {
+        R sx, cx, sy, cy;
+        RM<R>::sincospi(x, sx, cx);
+        RM<R>::sincospi(y, sy, cy);
+        gx = (R)3.141592653589793 * cx * sy + (R)2.0 * x;
+        gy = (R)3.141592653589793 * sx * cy + (R)2.0 * y;
+    } else {
+        R s, c;
+        RM<R>::sincos(y, s, c);
+        const R ex = RM<R>::exp(x);
+        gx = ex * c;
+        gy = -ex * s;
+    }
+    return (gx * nx + gy * ny) - varphi * pde_g<PDE, R>(x, y, P);
 }
 
 // Dirichlet data g = u (all shipped problems are manufactured solutions)

@@ -350,6 +350,9 @@ pdds_status pdds_server_create(pdds_engine *e, const pdds_server_config *cfg, pd
         pdds_status st = check_run(e, &c.run);
         if (!st) st = make_params(e, &c.problem, &c.run, s->pp);
         if (st) return st;
+        // a round carries every request's own stream and k2 (per-lane kernels), which the
+        // reflecting-side kernels do not take
+        if (s->pp.refl) return fail(e, PDDS_ERR_UNSUPPORTED, "serving with reflecting sides");
     }
     pdds_server *sp = s.get();
     for (int i = 0; i < c.prep_threads; ++i) sp->preps.emplace_back(prep_loop, sp);

@@ -23,10 +23,15 @@ class bvp:
     `u(x, y)` evaluates the manufactured solution on the host (for error reports).
     """
 
-    def __init__(self, pde="monegros", k2=0.0, lut=None):
+    def __init__(self, pde="monegros", k2=0.0, lut=None, neumann=0, varphi=0.0, rho=0.0):
+        """neumann: reflecting global sides (bits SIDE_N/S/E/W of _capi), where du/dn = varphi u + psi
+        (varphi <= 0; psi from the solution); rho: Lepingle step within this distance of them."""
         self.pde = _PDE_NAMES[pde] if isinstance(pde, str) else int(pde)
         self.k2 = float(k2)
         self.lut = lut
+        self.neumann = int(neumann)
+        self.varphi = float(varphi)
+        self.rho = float(rho)
         if self.pde == A.PDE_VARCOEF_LUT and lut is None:
             raise ValueError("varcoef_lut needs a LUT")
 
@@ -35,6 +40,9 @@ class bvp:
         p.pde = self.pde
         p.k2 = self.k2
         p.lut = self.lut.h if self.lut is not None else None
+        p.refl = self.neumann
+        p.robin = self.varphi
+        p.rho = self.rho
         return p
 
     def u(self, x, y):
@@ -338,7 +346,7 @@ class Server:
         self.h = h
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             A.lib().pdds_server_destroy(self.h)
             self.h = None
 

@@ -630,13 +630,13 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
         v.ox = x0 + 0.5 / v.sx; v.oy = y0 + 0.5 / v.sy;
     }
     // device layout (see LutView): replicated border of kLutPad texels below and
-    // kLutPad + 1 above in both directions; fields 0..3 in a float4 plane; field 4 as
-    // x-quads (nf == 5) or fields 4..7 per texel (nf > 5) in a second float4 plane
+    // kLutPad + 1 above in both directions; fields 0..3 in a float4 plane; field 4 alone
+    // (nf == 5, a float plane) or fields 4..7 per texel (nf > 5, a float4 plane) in plane b
     const int PX = NX + 2 * kLutPad + 1, PY = NY + 2 * kLutPad + 1;
     const size_t nt = (size_t)PX * PY;
     const bool has_b = nf > 4;
     if (nt * 16 >= ((size_t)1 << 32)) return fail(e, PDDS_ERR_UNSUPPORTED, "LUT larger than 4 GiB per plane");
-    std::vector<float> pa(nt * 4, 0.0f), pb(has_b ? nt * 4 : 4, 0.0f);
+    std::vector<float> pa(nt * 4, 0.0f), pb(nf > 5 ? nt * 4 : nf == 5 ? nt : 4, 0.0f);
     auto src = [&](int pj, int pi, int f) {
         const int j = std::min(std::max(pj - kLutPad, 0), NY - 1);
         const int i = std::min(std::max(pi - kLutPad, 0), NX - 1);
@@ -647,7 +647,7 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
             const size_t t = (size_t)pj * PX + pi;
             for (int f = 0; f < std::min(nf, 4); ++f) pa[t * 4 + f] = src(pj, pi, f);
             if (nf == 5)
-                for (int a = 0; a < 4; ++a) pb[t * 4 + a] = src(pj, std::min(pi + a, PX - 1), 4);
+                pb[t] = src(pj, pi, 4);
             else
                 for (int f = 4; f < nf; ++f) pb[t * 4 + (f - 4)] = src(pj, pi, f);
         }
@@ -666,7 +666,7 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
         return fail(e, PDDS_ERR_DEVICE, "LUT upload failed");
     }
     v.a = (const float4 *)l->d;
-    v.b = bytes_b ? (const float4 *)((const char *)l->d + bytes_a) : nullptr;
+    v.b = bytes_b ? (const void *)((const char *)l->d + bytes_a) : nullptr;
     l->view = v;
     l->nx = nx; l->ny = ny; l->nf = nf; l->mode = mode;
     *out = l;

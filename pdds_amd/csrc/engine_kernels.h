@@ -398,8 +398,14 @@ struct Walker {
             s.st += stepping ? 1 : 0;
         }
         R ex, ey, nx, ny;
-        int kind;
-        R d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
+        int kind = 0;
+        // the stopping test needs only the distance and the nearest side's normal; on a rectangle
+        // without reflecting sides those come without branches, and the exit projection and side
+        // are formed below for the stopping lanes alone
+        constexpr bool fast = GEOM == GeomRect && !REFL;
+        R d;
+        if constexpr (fast) d = rect_dist_inside<R>(rect, s.x, s.y, nx, ny);
+        else d = domain_dist<R, GEOM>(rect, s.x, s.y, ex, ey, nx, ny, kind);
         bool rnow = false;
         if constexpr (REFL) {
             // past a reflecting side: back onto it, the distance becoming local time for the next
@@ -424,6 +430,10 @@ struct Walker {
         const bool go = rnow || d < beta * nrm * sqrth;
         const bool capped = (int64_t)s.st >= a.max_steps;
         if (stepping && (!go || capped)) {
+            if constexpr (fast) {
+                R tx, ty;
+                (void)rect_dist<R>(rect, s.x, s.y, ex, ey, tx, ty, kind);
+            }
             write_exit<R, PDE, GEOM>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
             active = false;
         }
@@ -597,8 +607,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
 // LDS-tiled path kernel (coefficient-LUT problem): one work-group per CU.  The work-group
 // works on one patch at a time: it copies the patch's plane-a tile into LDS, its waves take
 // paths from the patch's queue until it is drained, and once every wave is done it moves to
-// the next patch with paths left.  The 16 taps of a step are then LDS reads; only the
-// x-quads of field 4 (4 per step) and taps outside the tile go to the vector L1.
+// the next patch with paths left.  The 16 taps of a step are then LDS reads; only field 4
+// (one 16-byte load per tap row, 4 per step, from its float plane) and taps outside the tile
+// go to the vector L1.
 // ----------------------------------------------------------------------------
 // one work-group per CU holding every wave the register budget allows (the run-time step-family
 // variants one wave less, as in path_kernel)

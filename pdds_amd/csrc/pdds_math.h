@@ -412,6 +412,23 @@ PDDS_HD R rect_dist(const R p[4], R x, R y, R &ex, R &ey, R &nx, R &ny, int &kin
     return dy - hy;
 }
 
+// What the stopping test needs of rect_dist, without branches: inside the rectangle its signed
+// distance and outward normal, bit for bit (same operations, same tie rule); outside a positive
+// distance (the largest side excess, which is not rect_dist's corner distance) -- a point there
+// stops whatever its value, and the exit projection comes from rect_dist itself.
+template <class R>
+PDDS_HD R rect_dist_inside(const R p[4], R x, R y, R &nx, R &ny) {
+    const R cx = (R)0.5 * (p[0] + p[2]), cy = (R)0.5 * (p[1] + p[3]);
+    const R hx = p[2] - cx, hy = p[3] - cy;
+    const R ax = RM<R>::fabs(x - cx) - hx, ay = RM<R>::fabs(y - cy) - hy;
+    const bool xside = ax > ay;
+    const R sxs = x > cx ? (R)1 : (R)-1;
+    const R sys = y > cy ? (R)1 : (R)-1;
+    nx = xside ? sxs : (R)0;
+    ny = xside ? (R)0 : sys;
+    return xside ? ax : ay;
+}
+
 // Signed distance to the annular sector g = {r0, r1, cos a, sin a, cos b, sin b}: radii
 // r0 <= r <= r1 between the rays a and b (b counterclockwise from a, less than pi apart;
 // r0 = 0 for a sector at the centre, which then has no inner side; r0 < 0 for the whole disc
@@ -464,16 +481,21 @@ PDDS_HD R sector_dist(const R g[6], R x, R y, R &ex, R &ey, R &nx, R &ny, int &k
 // texels), so every tap of a clamped coordinate t in [-2, nx + 1] lies inside the
 // padded table and no per-tap clamp is needed; the replicated border gives exactly the
 // clamp-to-edge values.  Fields 0..3 live in one float4 plane (a).  For nf == 5, plane b
-// holds field 4 as x-quads: entry t = (f4[t], f4[t+1], f4[t+2], f4[t+3]) along the padded
-// row, so the four taps of a row are one 16-byte load; for nf > 5 plane b holds fields
-// 4..7 per texel.  Every load is at a 32-bit byte offset from a uniform base.  A gather of
-// scattered lanes costs one cache line per lane and instruction, so a step costs 16 + 4
-// line accesses per lane for nf == 5 (16 + 16 for nf > 5).
+// holds field 4 alone, one float per texel, and the four taps of a row are one 16-byte load
+// at a 4-byte aligned offset (a row's taps never run past the padded row: the last tap
+// column is at most nx + 6 = px - 1); a patch's field-4 texels then take a quarter of the
+// L2 that 16-byte x-quads would.  For nf > 5 plane b holds fields 4..7 per texel.  Every
+// load is at a 32-bit byte offset from a uniform base.  A gather of scattered lanes costs
+// one cache line per lane and instruction, so a step costs 16 + 4 line accesses per lane
+// for nf == 5 (16 + 16 for nf > 5).
 constexpr int kLutPad = 3;
+
+// four consecutive floats of a 4-byte aligned address
+typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 struct LutView {
     const float4 *a;       // fields 0..3
-    const float4 *b;       // nf == 5: x-quads of field 4; nf > 5: fields 4..7; else null
+    const void *b;         // nf == 5: field 4 (float per texel); nf > 5: fields 4..7 (float4); else null
     int nx, ny, px, nf;    // logical texels per row / column, padded row length, fields
     double ox, oy, sx, sy;
 };
@@ -515,12 +537,16 @@ PDDS_HD void lut_accum(const LutView &L, const LutTile &T, lds_float4 *tp, bool 
                        const float wy[4], float acc[NF]) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = 0.0f;
-    // the fifth field's x-quads of the four rows first, so that their global loads are all in
+    // the fifth field's four taps of each row first, so that their global loads are all in
     // flight while the rows' plane-a taps are read
-    float4 Q[4];
+    float4_a4 Q[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
-        Q[b] = NF == 5 ? lut_at<float4>(L.b, (t0 + (uint32_t)b * (uint32_t)L.px) * 16u) : make_float4(0.f, 0.f, 0.f, 0.f);
+        Q[b] = NF == 5 ? lut_at<float4_a4>(L.b, (t0 + (uint32_t)b * (uint32_t)L.px) * 4u) : float4_a4{0.f, 0.f, 0.f, 0.f};
+    // fields 0..3 (and 4..7 for eight fields) row by row; the fifth field's quads last, so that
+    // their global loads have the whole plane-a accumulation to arrive (each field's fused
+    // multiply-adds keep their order, so the values do not depend on this)
+    constexpr int NA = NF == 5 ? 4 : NF;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const uint32_t r0 = t0 + (uint32_t)b * (uint32_t)L.px;
@@ -532,27 +558,34 @@ PDDS_HD void lut_accum(const LutView &L, const LutTile &T, lds_float4 *tp, bool 
 #pragma unroll
             for (int a = 0; a < 4; ++a) A[a] = lut_at<float4>(L.a, (r0 + (uint32_t)a) * 16u);
         }
-        float row[NF];
+        float row[NA];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) row[f] = 0.0f;
-        const float4 quad = Q[b];
-        const float f4[4] = {quad.x, quad.y, quad.z, quad.w};
+        for (int f = 0; f < NA; ++f) row[f] = 0.0f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             float c[8];
             const float4 q0 = A[a];
             c[0] = q0.x; c[1] = q0.y; c[2] = q0.z; c[3] = q0.w;
-            if constexpr (NF == 5) {
-                c[4] = f4[a];
-            } else if constexpr (NF > 5) {
+            if constexpr (NF > 5) {
                 const float4 q1 = lut_at<float4>(L.b, (r0 + (uint32_t)a) * 16u);
                 c[4] = q1.x; c[5] = q1.y; c[6] = q1.z; c[7] = q1.w;
             }
 #pragma unroll
-            for (int f = 0; f < NF; ++f) row[f] = __builtin_fmaf(wx[a], c[f], row[f]);
+            for (int f = 0; f < NA; ++f) row[f] = __builtin_fmaf(wx[a], c[f], row[f]);
         }
 #pragma unroll
-        for (int f = 0; f < NF; ++f) acc[f] = __builtin_fmaf(wy[b], row[f], acc[f]);
+        for (int f = 0; f < NA; ++f) acc[f] = __builtin_fmaf(wy[b], row[f], acc[f]);
+    }
+    if constexpr (NF == 5) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float4_a4 quad = Q[b];
+            float row = __builtin_fmaf(wx[0], quad.x, 0.0f);
+            row = __builtin_fmaf(wx[1], quad.y, row);
+            row = __builtin_fmaf(wx[2], quad.z, row);
+            row = __builtin_fmaf(wx[3], quad.w, row);
+            acc[4] = __builtin_fmaf(wy[b], row, acc[4]);
+        }
     }
 }
 

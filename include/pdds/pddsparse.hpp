@@ -44,8 +44,10 @@ struct bvp {
     int32_t neumann = 0;     // reflecting global sides (PDDS_SIDE_* bits), the reference's bvp.Neumann
     double varphi = 0.0;     // Robin coefficient on those sides (<= 0)
     double rho = 0.0;        // Lepingle step within this distance of a reflecting side (0: always)
+    const pdds_lut *ulut = nullptr;   // PDDS_PDE_TABLE: u, the Dirichlet data (1 field)
+    const pdds_lut *vlut = nullptr;   // PDDS_STEP_VRCV: mu1, mu2, F1, F2
     std::function<double(double, double)> u;   // analytic solution (solution.csv); optional
-    pdds_problem desc() const { pdds_problem p{pde, k2, lut, neumann, varphi, rho}; return p; }
+    pdds_problem desc() const { pdds_problem p{pde, k2, lut, neumann, varphi, rho, ulut, vlut}; return p; }
 };
 
 // Owns one engine (one device).

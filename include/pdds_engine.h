@@ -62,6 +62,11 @@ pdds_status pdds_engine_info(const pdds_engine *e, int32_t *n_cu, char *name, in
 #define PDDS_PDE_VARCOEF 2       /* div form 1/2 div(s s^T grad u) + b.grad u - c u = f, u = e^x cos y */
 #define PDDS_PDE_VARCOEF_LUT 3   /* PDE 2 with sigma11, sigma22, b1, b2, c_ref from a 5-field LUT */
 #define PDDS_PDE_HELMHOLTZ 4     /* u as PDE 1, c = -k2 */
+#define PDDS_PDE_TABLE 5         /* every coefficient tabulated: lut holds sigma11, sigma22, b1, b2, c, f,
+                                    du/dx, du/dy (8 fields; sigma diagonal) and ulut the solution u, the
+                                    Dirichlet data g (1 field): the texture-served problem of the
+                                    reference's MCinCUDA texMode 4 (Solvers/MCinCUDA.cu:453-478), which
+                                    serves f, c and grad u from textures, with sigma and b tabulated too */
 /* Boundary sides of the global rectangle, the bits of pdds_problem.refl */
 #define PDDS_SIDE_N 1
 #define PDDS_SIDE_S 2
@@ -83,6 +88,8 @@ typedef struct {
     int32_t refl;
     double robin;
     double rho;
+    const pdds_lut *ulut;   /* PDDS_PDE_TABLE: u (1 field) */
+    const pdds_lut *vlut;   /* PDDS_STEP_VRCV: mu1, mu2, F1, F2 (4 fields) */
 } pdds_problem;
 
 /* ---- run parameters ---- */
@@ -95,6 +102,11 @@ typedef struct {
 #define PDDS_STEP_VR 2          /* variance reduction by a change of drift: Y += Y mu . dW and
                                    X += (b - sigma mu) h + sigma dW, xi = 0 (FKACIntegrator.hpp:31-44);
                                    u must not vanish along the paths */
+#define PDDS_STEP_VRCV 3        /* variance reduction and control variate with mu and F from the
+                                   problem's vlut: xi += Y F . dW, Y += c Y h + Y mu . dW and
+                                   X += (b - sigma mu) h + sigma dW (FKACIntegrator.hpp:100-112); any
+                                   bounded mu and F leave E[score] unchanged, and F = -sigma^T grad u
+                                   - u mu makes the score's martingale part vanish */
 #define PDDS_FP64 0             /* FP64 path state; bit-exact with the FP64 oracle */
 #define PDDS_FP32 1             /* FP32 path state, Kahan functionals, FP64 cross-path sums */
 typedef struct {

@@ -186,10 +186,10 @@ static void orc_bspline_wf(float f, float w[4]) {
    coefficients in FP32 with fma -- the arithmetic of the reference's FP32 texture path
    (Integrators/cubicTex2D.cu:64-87, Integrators/bspline_kernel.cu:52-60).  lut_nf is the
    texel stride in floats. */
-double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
-    const int nx = pb->lut_nx, ny = pb->lut_ny, nf = pb->lut_nf;
-    double t = (x - pb->lut_ox) * pb->lut_sx;
-    double v = (y - pb->lut_oy) * pb->lut_sy;
+static double orc_lut_at(const orc_lut *L, int field, double x, double y) {
+    const int nx = L->nx, ny = L->ny, nf = L->nf;
+    double t = (x - L->ox) * L->sx;
+    double v = (y - L->oy) * L->sy;
     t = fmin(fmax(t, -2.0), (double)nx + 1.0);
     v = fmin(fmax(v, -2.0), (double)ny + 1.0);
     double fi = floor(t), fj = floor(v);
@@ -203,11 +203,16 @@ double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
         float row = 0.0f;
         for (int a = 0; a < 4; ++a) {
             int ii = i - 1 + a; ii = ii < 0 ? 0 : (ii > nx - 1 ? nx - 1 : ii);
-            row = fmaf(wx[a], pb->lut[((int64_t)jj * nx + ii) * nf + field], row);
+            row = fmaf(wx[a], L->c[((int64_t)jj * nx + ii) * nf + field], row);
         }
         acc = fmaf(wy[b], row, acc);
     }
     return (double)acc;
+}
+
+double orc_lut_eval(const orc_problem *pb, int field, double x, double y) {
+    const orc_lut L = {pb->lut, pb->lut_nx, pb->lut_ny, pb->lut_nf, pb->lut_ox, pb->lut_oy, pb->lut_sx, pb->lut_sy};
+    return orc_lut_at(&L, field, x, y);
 }
 
 /* Coefficients at (x,y): sigma (row-major s00 s01 s10 s11), b, c, f, grad u, g, u */
@@ -236,6 +241,21 @@ void orc_pde_eval(const orc_problem *pb, double x, double y, double sig[4], doub
             if (pb->pde == ORC_PDE_SINSIN) { *c = 0.0; *f = flap; }
             else { *c = -pb->k2; *f = flap + pb->k2 * uu; }
             *g = uu; *u = uu;
+            break;
+        }
+        case ORC_PDE_TABLE: {
+            /* the texture-served problem (Solvers/MCinCUDA.cu:453-478, texMode 4: f, c and
+               grad u from textures), sigma and b tabulated as well */
+            sig[0] = orc_lut_eval(pb, 0, x, y); sig[1] = 0.0; sig[2] = 0.0;
+            sig[3] = orc_lut_eval(pb, 1, x, y);
+            b[0] = orc_lut_eval(pb, 2, x, y);
+            b[1] = orc_lut_eval(pb, 3, x, y);
+            *c = orc_lut_eval(pb, 4, x, y);
+            *f = orc_lut_eval(pb, 5, x, y);
+            grad[0] = orc_lut_eval(pb, 6, x, y);
+            grad[1] = orc_lut_eval(pb, 7, x, y);
+            *g = orc_lut_at(&pb->ulut, 0, x, y);
+            *u = *g;
             break;
         }
         default: { /* VARCOEF and VARCOEF_LUT */
@@ -269,7 +289,7 @@ void orc_pde_eval(const orc_problem *pb, double x, double y, double sig[4], doub
 /* ------------------------------------------------------------------------ */
 /* Monte Carlo: EMFKACSolver::Simulate_OMP (Solvers/EMFKACSolver.hpp:108-216) */
 /* ------------------------------------------------------------------------ */
-typedef struct { double x, y, Y, Z, xi; double sig[4], b[2], c, f, grad[2]; } orc_state;
+typedef struct { double x, y, Y, Z, xi; double sig[4], b[2], c, f, grad[2], mu[2], F[2]; } orc_state;
 
 static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
     double g, u;
@@ -280,6 +300,11 @@ static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
     } else if (pb->step == 2) {
         s->grad[0] = s->grad[0] / g;   /* g = u for every manufactured problem */
         s->grad[1] = s->grad[1] / g;
+    } else if (pb->step == 3) {
+        s->mu[0] = orc_lut_at(&pb->vlut, 0, s->x, s->y);
+        s->mu[1] = orc_lut_at(&pb->vlut, 1, s->x, s->y);
+        s->F[0] = orc_lut_at(&pb->vlut, 2, s->x, s->y);
+        s->F[1] = orc_lut_at(&pb->vlut, 3, s->x, s->y);
     }
 }
 
@@ -287,7 +312,8 @@ static inline void orc_coeffs(const orc_problem *pb, orc_state *s) {
    psi = varphi = 0.  Coefficients at the old X are already in s, with v = -sigma^T grad.
    Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29):
    xi += Y v . dW.  VR step (:31-44, grad holds grad u / u so v = mu): Y += c Y h + Y mu . dW
-   and X's drift is b - sigma mu. */
+   and X's drift is b - sigma mu.  VR+CV step (:100-112, mu and F from the table): xi += Y F . dW,
+   Y += c Y h + Y mu . dW, X's drift b - sigma mu. */
 static inline void orc_step(int step, orc_state *s, double h, double sqrth, double n1, double n2, double lz,
                             double ly) {
     /* lz = psi Y L, ly = varphi Y L: the local-time terms of a reflecting side (0 elsewhere),
@@ -295,6 +321,7 @@ static inline void orc_step(int step, orc_state *s, double h, double sqrth, doub
     double i0 = sqrth * n1, i1 = sqrth * n2;
     double v0 = -(s->sig[0] * s->grad[0] + s->sig[2] * s->grad[1]);
     double v1 = -(s->sig[1] * s->grad[0] + s->sig[3] * s->grad[1]);
+    if (step == 3) { v0 = s->F[0]; v1 = s->F[1]; }
     double dxi = s->Y * (v0 * i0 + v1 * i1);
     if (step != 2) s->xi = s->xi + dxi;
     s->Z = s->Z + (s->f * s->Y * h + lz);
@@ -304,6 +331,11 @@ static inline void orc_step(int step, orc_state *s, double h, double sqrth, doub
         s->Y = Yold + ((s->c * Yold * h + dxi) + ly);
         b0 = b0 - (s->sig[0] * v0 + s->sig[1] * v1);
         b1 = b1 - (s->sig[2] * v0 + s->sig[3] * v1);
+    } else if (step == 3) {
+        const double m0 = s->mu[0], m1 = s->mu[1];
+        s->Y = Yold + ((s->c * Yold * h + Yold * (m0 * i0 + m1 * i1)) + ly);
+        b0 = b0 - (s->sig[0] * m0 + s->sig[1] * m1);
+        b1 = b1 - (s->sig[2] * m0 + s->sig[3] * m1);
     } else {
         s->Y = Yold + (s->c * Yold * h + ly);
     }

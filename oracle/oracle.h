@@ -34,8 +34,18 @@ enum {
     ORC_PDE_SINSIN = 1,       /* u = sin(pi x) sin(pi y) + x^2 + y^2, Poisson */
     ORC_PDE_VARCOEF = 2,      /* divergence form, u = e^x cos y, analytic coefficients */
     ORC_PDE_VARCOEF_LUT = 3,  /* same PDE, sigma/b/c from a 5-field B-spline LUT */
-    ORC_PDE_HELMHOLTZ = 4     /* u as SINSIN, c = -k^2 */
+    ORC_PDE_HELMHOLTZ = 4,    /* u as SINSIN, c = -k^2 */
+    ORC_PDE_TABLE = 5         /* every coefficient from the 8-field LUT (sigma11, sigma22, b1, b2,
+                                 c, f, du/dx, du/dy), u = g from the 1-field ulut */
 };
+
+/* A B-spline LUT on its own: coefficient array [ny][nx][nf] (float), t = (x - ox) * sx,
+   u = (y - oy) * sy */
+typedef struct {
+    const float *c;
+    int nx, ny, nf;
+    double ox, oy, sx, sy;
+} orc_lut;
 
 typedef struct {
     int pde;
@@ -47,12 +57,15 @@ typedef struct {
     double lut_ox, lut_oy, lut_sx, lut_sy;
     int step;               /* step family: 0 control variate (Integrators/FKACIntegrator.hpp:46-57),
                                1 plain, grad u taken as 0 (:20-29), 2 VR, the drift change by
-                               mu = -sigma^T grad u / u (:31-44) */
+                               mu = -sigma^T grad u / u (:31-44), 3 VR+CV with mu and F from
+                               vlut (:100-112) */
     /* reflecting sides of the stopping rectangle (bit 0 N, 1 S, 2 E, 3 W): the Lepingle step
        (Integrators/FKACIntegrator.hpp:114-165) with du/dn = robin u + psi, psi from the
        solution; rho: Lepingle step within this distance (0: always) */
     int refl;
     double robin, rho;
+    orc_lut ulut;           /* TABLE: u (1 field) */
+    orc_lut vlut;           /* step 3: mu1, mu2, F1, F2 */
 } orc_problem;
 
 typedef struct {

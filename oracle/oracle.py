@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-PDE_MONEGROS, PDE_SINSIN, PDE_VARCOEF, PDE_VARCOEF_LUT, PDE_HELMHOLTZ = range(5)
+PDE_MONEGROS, PDE_SINSIN, PDE_VARCOEF, PDE_VARCOEF_LUT, PDE_HELMHOLTZ, PDE_TABLE = range(6)
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int)
@@ -23,12 +23,18 @@ _u32 = C.POINTER(C.c_uint32)
 _f = C.POINTER(C.c_float)
 
 
+class Lut(C.Structure):
+    _fields_ = [("c", _f), ("nx", C.c_int), ("ny", C.c_int), ("nf", C.c_int),
+                ("ox", C.c_double), ("oy", C.c_double), ("sx", C.c_double), ("sy", C.c_double)]
+
+
 class Problem(C.Structure):
     _fields_ = [("pde", C.c_int), ("k2", C.c_double), ("lut", _f),
                 ("lut_nx", C.c_int), ("lut_ny", C.c_int), ("lut_nf", C.c_int),
                 ("lut_ox", C.c_double), ("lut_oy", C.c_double),
                 ("lut_sx", C.c_double), ("lut_sy", C.c_double), ("step", C.c_int),
-                ("refl", C.c_int), ("robin", C.c_double), ("rho", C.c_double)]
+                ("refl", C.c_int), ("robin", C.c_double), ("rho", C.c_double),
+                ("ulut", Lut), ("vlut", Lut)]
 
 
 class Run(C.Structure):
@@ -122,20 +128,43 @@ def sector_dist(g, x, y):
     return (d, o[0], o[1], o[2], o[3], int(k[0]))
 
 
-def problem(pde, k2=0.0, lut=None, step=0, refl=0, robin=0.0, rho=0.0):
-    """step: 0 control variate, 1 plain, 2 VR (the engine's pdds_run.step values); refl: reflecting
-    sides of the stopping rectangle (bits N, S, E, W), robin: varphi there, rho: Lepingle band."""
+def problem(pde, k2=0.0, lut=None, step=0, refl=0, robin=0.0, rho=0.0, ulut=None, vlut=None):
+    """step: 0 control variate, 1 plain, 2 VR, 3 VR+CV (the engine's pdds_run.step values); refl:
+    reflecting sides of the stopping rectangle (bits N, S, E, W), robin: varphi there, rho: Lepingle
+    band; ulut: u of PDE_TABLE, vlut: mu and F of the VR+CV step (lut_dict dictionaries)."""
     pb = Problem()
     pb.pde = pde
     pb.k2 = k2
     pb.step = step
     pb.refl, pb.robin, pb.rho = refl, robin, rho
+    pb._keep = []
     if lut is not None:
-        pb._lut_keep = lut["coef"]
+        pb._keep.append(lut["coef"])
         pb.lut = ptr(lut["coef"], _f)
         pb.lut_nx, pb.lut_ny, pb.lut_nf = lut["NX"], lut["NY"], lut["nf"]
         pb.lut_ox, pb.lut_oy, pb.lut_sx, pb.lut_sy = lut["ox"], lut["oy"], lut["sx"], lut["sy"]
+    for name, d in (("ulut", ulut), ("vlut", vlut)):
+        if d is not None:
+            pb._keep.append(d["coef"])
+            setattr(pb, name, Lut(ptr(d["coef"], _f), d["NX"], d["NY"], d["nf"], d["ox"], d["oy"], d["sx"], d["sy"]))
     return pb
+
+
+def lut_dict(samples, box, mode=0):
+    """Oracle LUT of samples [ny][nx][nf] on box (x0, y0, x1, y1), with the engine's coordinate map:
+    mode 0 prefiltered node-aligned samples (interpolating), mode 1 raw texel-centred samples."""
+    s = np.asarray(samples, dtype=np.float64)
+    if s.ndim == 2:
+        s = s[:, :, None]
+    ny, nx, nf = s.shape
+    x0, y0, x1, y1 = box
+    if mode == 0:
+        dx, dy = (x1 - x0) / (nx - 1), (y1 - y0) / (ny - 1)
+        return {"coef": lut_prefilter(s), "NX": nx + 2, "NY": ny + 2, "nf": nf, "ox": x0 - dx, "oy": y0 - dy,
+                "sx": 1.0 / dx, "sy": 1.0 / dy}
+    sx, sy = nx / (x1 - x0), ny / (y1 - y0)
+    return {"coef": np.ascontiguousarray(s, dtype=np.float32), "NX": nx, "NY": ny, "nf": nf,
+            "ox": x0 + 0.5 / sx, "oy": y0 + 0.5 / sy, "sx": sx, "sy": sy}
 
 
 def run_params(h, seed, beta=-0.5826, stream=0, normals=0, phase2=0, max_steps=0, threads=0):

@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PDDS_LIB") or os.path.join(_HERE, "libpdds.so")
 
 OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_SINGULAR, ERR_NOT_READY, ERR_FULL = range(8)
-PDE_MONEGROS, PDE_SINSIN, PDE_VARCOEF, PDE_VARCOEF_LUT, PDE_HELMHOLTZ = range(5)
+PDE_MONEGROS, PDE_SINSIN, PDE_VARCOEF, PDE_VARCOEF_LUT, PDE_HELMHOLTZ, PDE_TABLE = range(6)
+STEP_CV, STEP_PLAIN, STEP_VR, STEP_VRCV = range(4)
 FP64, FP32 = 0, 1
 
 _d = C.POINTER(C.c_double)
@@ -22,7 +23,7 @@ _i64 = C.POINTER(C.c_int64)
 
 class Problem(C.Structure):
     _fields_ = [("pde", C.c_int32), ("k2", C.c_double), ("lut", C.c_void_p), ("refl", C.c_int32),
-                ("robin", C.c_double), ("rho", C.c_double)]
+                ("robin", C.c_double), ("rho", C.c_double), ("ulut", C.c_void_p), ("vlut", C.c_void_p)]
 
 
 SIDE_N, SIDE_S, SIDE_E, SIDE_W = 1, 2, 4, 8

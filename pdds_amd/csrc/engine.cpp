@@ -88,7 +88,7 @@ pdds_status dbuf(pdds_engine *e, int slot, size_t bytes, void **out) {
     return PDDS_OK;
 }
 
-bool valid_pde(int pde) { return pde >= PDDS_PDE_MONEGROS && pde <= PDDS_PDE_HELMHOLTZ; }
+bool valid_pde(int pde) { return pde >= PDDS_PDE_MONEGROS && pde <= PDDS_PDE_TABLE; }
 
 pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r, PdeParams &pp) {
     if (!p || !valid_pde(p->pde)) return fail(e, PDDS_ERR_INVALID, "invalid problem descriptor");
@@ -103,18 +103,34 @@ pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r
     pp.robin = p->robin;
     pp.rho = p->rho;
     if (p->pde == PDDS_PDE_VARCOEF_LUT) {
-        if (!p->lut || p->lut->nf < 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
+        if (!p->lut || p->lut->nf != 5) return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_VARCOEF_LUT needs a LUT with 5 fields");
         pp.lut = p->lut->view;
+    }
+    if (p->pde == PDDS_PDE_TABLE) {
+        if (!p->lut || p->lut->nf != 8 || !p->ulut || p->ulut->nf != 1)
+            return fail(e, PDDS_ERR_INVALID, "PDDS_PDE_TABLE needs an 8-field lut and a 1-field ulut");
+        pp.lut = p->lut->view;
+        pp.ulut = p->ulut->view;
+    }
+    if (pp.step == STEP_VRCV) {
+        if (!p->vlut || p->vlut->nf != 4) return fail(e, PDDS_ERR_INVALID, "PDDS_STEP_VRCV needs a 4-field vlut");
+        pp.vlut = p->vlut->view;
     }
     return PDDS_OK;
 }
 
-// host evaluation of the Dirichlet data (boundary knots: B = g(X0), Solvers/EMFKACSolver.hpp:720-727)
-double host_g(int pde, double x, double y, const PdeParams &pp) {
-    switch (pde) {
+// host evaluation of the Dirichlet data (boundary knots: B = g(X0), Solvers/EMFKACSolver.hpp:720-727);
+// a tabulated u is read from the host copy of its table
+double host_g(const pdds_problem *p, double x, double y, const PdeParams &pp) {
+    switch (p->pde) {
         case PDE_MONEGROS: return pde_g<PDE_MONEGROS, double>(x, y, pp);
         case PDE_SINSIN: return pde_g<PDE_SINSIN, double>(x, y, pp);
         case PDE_HELMHOLTZ: return pde_g<PDE_HELMHOLTZ, double>(x, y, pp);
+        case PDE_TABLE: {
+            PdeParams hp = pp;
+            hp.ulut = p->ulut->hview;
+            return pde_g<PDE_TABLE, double>(x, y, hp);
+        }
         default: return pde_g<PDE_VARCOEF, double>(x, y, pp);
     }
 }
@@ -129,9 +145,10 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
         return fail(e, PDDS_ERR_INVALID, "negative launch parameter");
     if (r->paths_per_lane < 0 || r->paths_per_lane > 64)
         return fail(e, PDDS_ERR_INVALID, "paths_per_lane must be in [0, 64]");
-    static_assert(PDDS_STEP_CV == STEP_CV && PDDS_STEP_PLAIN == STEP_PLAIN && PDDS_STEP_VR == STEP_VR,
-                  "step family values");
-    if (r->step < PDDS_STEP_CV || r->step > PDDS_STEP_VR) return fail(e, PDDS_ERR_INVALID, "bad step family");
+    static_assert(PDDS_STEP_CV == STEP_CV && PDDS_STEP_PLAIN == STEP_PLAIN && PDDS_STEP_VR == STEP_VR &&
+                      PDDS_STEP_VRCV == STEP_VRCV && PDDS_PDE_TABLE == PDE_TABLE,
+                  "step family and problem values");
+    if (r->step < PDDS_STEP_CV || r->step > PDDS_STEP_VRCV) return fail(e, PDDS_ERR_INVALID, "bad step family");
     return PDDS_OK;
 }
 
@@ -655,6 +672,12 @@ pdds_status pdds_lut_create(pdds_engine *e, int32_t nx, int32_t ny, int32_t nf, 
     pdds_lut *l = new pdds_lut();
     l->e = e;
     const size_t bytes_a = pa.size() * sizeof(float), bytes_b = has_b ? pb.size() * sizeof(float) : 0;
+    l->host.resize((bytes_a + bytes_b) / sizeof(float));
+    std::memcpy(l->host.data(), pa.data(), bytes_a);
+    if (bytes_b) std::memcpy(l->host.data() + pa.size(), pb.data(), bytes_b);
+    l->hview = v;
+    l->hview.a = (const float4 *)l->host.data();
+    l->hview.b = bytes_b ? (const void *)(l->host.data() + pa.size()) : nullptr;
     if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&l->d, bytes_a + bytes_b) != hipSuccess) {
         delete l;
         return fail(e, PDDS_ERR_NOMEM, "LUT allocation failed");
@@ -869,7 +892,7 @@ static pdds_status assemble_locked(pdds_engine *e, const pdds_mesh *m, const pdd
                     return fail(e, PDDS_ERR_UNSUPPORTED, "a knot lies on a reflecting side (knot " + std::to_string(q) + ")");
         }
         if (M.on_boundary[q]) {
-            B[q] = host_g(p->pde, M.x0[q], M.y0[q], pp);
+            B[q] = host_g(p, M.x0[q], M.y0[q], pp);
             BB[q] = B[q] * B[q];
             if (eq_mean) eq_mean[q] = B[q];
             if (eq_var) eq_var[q] = 0.0;
@@ -1184,7 +1207,7 @@ pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_probl
     PdeParams pp;
     if ((st = make_params(e, p, r, pp))) return st;
     if (M.on_boundary[knot]) {
-        *B = host_g(p->pde, M.x0[knot], M.y0[knot], pp);
+        *B = host_g(p, M.x0[knot], M.y0[knot], pp);
         *BB = *B * *B;
         if (ctr) std::memset(ctr, 0, sizeof(*ctr));
         return PDDS_OK;
@@ -1229,7 +1252,7 @@ static pdds_status point_eval_locked(pdds_engine *e, const pdds_problem *p, cons
         double dist = rect_dist<double>(glob, x[i], y[i], ex, ey, nx, ny, kd);
         if (std::fabs(dist) <= 1e-8 || dist > 0.0) {
             // on (or outside) the boundary: the Dirichlet value, no Monte Carlo
-            u[i] = host_g(p->pde, ex, ey, pp);
+            u[i] = host_g(p, ex, ey, pp);
             if (u_var) u_var[i] = 0.0;
             continue;
         }

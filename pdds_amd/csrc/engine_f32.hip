@@ -47,6 +47,7 @@ hipError_t launch_path_f32(int pde, int geom, bool persistent, bool mixed, int g
         case PDE_VARCOEF: return path_f32<PDE_VARCOEF>(geom, persistent, mixed, grid, wg, s, a);
         case PDE_VARCOEF_LUT: return path_f32<PDE_VARCOEF_LUT>(geom, persistent, mixed, grid, wg, s, a);
         case PDE_HELMHOLTZ: return path_f32<PDE_HELMHOLTZ>(geom, persistent, mixed, grid, wg, s, a);
+        case PDE_TABLE: return path_f32<PDE_TABLE>(geom, persistent, mixed, grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }
@@ -92,6 +93,7 @@ hipError_t launch_paths_f32(int pde, int grid, int wg, hipStream_t s, const Path
         case PDE_VARCOEF: return paths_f32<PDE_VARCOEF>(grid, wg, s, a);
         case PDE_VARCOEF_LUT: return paths_f32<PDE_VARCOEF_LUT>(grid, wg, s, a);
         case PDE_HELMHOLTZ: return paths_f32<PDE_HELMHOLTZ>(grid, wg, s, a);
+        case PDE_TABLE: return paths_f32<PDE_TABLE>(grid, wg, s, a);
         default: return hipErrorInvalidValue;
     }
 }

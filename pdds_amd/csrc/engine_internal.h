@@ -72,6 +72,8 @@ struct pdds_lut {
     pdds_engine *e = nullptr;
     float *d = nullptr;
     LutView view{};
+    std::vector<float> host;   // the device planes, for host evaluation (Dirichlet data of boundary knots)
+    LutView hview{};           // view with host pointers into `host`
     int nx = 0, ny = 0, nf = 0, mode = 0;
 };
 
@@ -115,7 +117,7 @@ pdds_status upload(pdds_engine *e, int slot, const std::vector<T> &v, T **out) {
 
 bool valid_pde(int pde);
 pdds_status make_params(pdds_engine *e, const pdds_problem *p, const pdds_run *r, PdeParams &pp);
-double host_g(int pde, double x, double y, const PdeParams &pp);
+double host_g(const pdds_problem *p, double x, double y, const PdeParams &pp);
 pdds_status check_run(pdds_engine *e, const pdds_run *r);
 
 // One generic assembly over a list of knots sharing the stencil table.

@@ -27,10 +27,11 @@
 // VGPRs).  FP64: the LUT kernels keep a step's LUT reads in flight with up to 256 VGPRs (2
 // waves), as does the Monegros problem (three transcendental chains per step); the other
 // variable-coefficient problem fits 3 waves and the Poisson/Helmholtz problems 4.  FP32: 3
-// waves for the LUT problem, 4 for the others.
+// waves for the LUT problem, 2 for the tabulated problem (eight fields per lookup), 4 for the
+// others.
 template <class R, int PDE> constexpr int path_waves() {
-    if constexpr (sizeof(R) == 4) return PDE == pdds::PDE_VARCOEF_LUT ? 3 : 4;
-    else if constexpr (PDE == pdds::PDE_VARCOEF_LUT || PDE == pdds::PDE_MONEGROS) return 2;
+    if constexpr (sizeof(R) == 4) return PDE == pdds::PDE_TABLE ? 2 : PDE == pdds::PDE_VARCOEF_LUT ? 3 : 4;
+    else if constexpr (PDE == pdds::PDE_VARCOEF_LUT || PDE == pdds::PDE_MONEGROS || PDE == pdds::PDE_TABLE) return 2;
     else if constexpr (PDE == pdds::PDE_VARCOEF) return 3;
     else return 4;
 }
@@ -99,14 +100,18 @@ template <class R> struct PathState {
 // Control-variate and plain steps (Integrators/FKACIntegrator.hpp:46-57, :20-29; the plain
 // step's g is 0): xi += Y v . dW with v = -sigma^T g.  VR step (FKACIntegrator.hpp:31-44; g is
 // grad u / u, so v is mu): Y += Y mu . dW and the drift of X loses sigma mu; xi is not used.
+// VR+CV step (FKACIntegrator.hpp:100-112, mu and F from the problem's table): xi += Y F . dW,
+// Y += c Y h + Y mu . dW and the drift of X loses sigma mu.
 // LT: the step carries local-time terms lz = psi Y L and ly = varphi Y L of a reflecting side
 // (Integrators/FKACIntegrator.hpp:156-158: Z += f Y h + psi Y L, Y += c Y h + varphi Y L).
 template <class R, int FAM, bool LT = false>
 __device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h, R sqrth, R n1, R n2, int fam,
                                         R lz = (R)0, R ly = (R)0) {
     R i0 = sqrth * n1, i1 = sqrth * n2;
+    const bool vrcv = FAM != STEP_CV && fam == STEP_VRCV;
     R v0 = -(co.s00 * co.g0 + co.s10 * co.g1);
     R v1 = -(co.s01 * co.g0 + co.s11 * co.g1);
+    if (vrcv) { v0 = co.F0; v1 = co.F1; }
     R dxi = s.Y * (v0 * i0 + v1 * i1);
     R dz = co.f * s.Y * h;
     if constexpr (LT) dz = dz + lz;
@@ -121,12 +126,14 @@ __device__ __forceinline__ void em_step(PathState<R> &s, const Coef<R> &co, R h,
     }
     R Yold = s.Y;
     R b0 = co.b0, b1 = co.b1;
-    if (vr) {
-        R dy = co.c * Yold * h + dxi;
+    if (vr || vrcv) {
+        // mu: v for VR, the table's for VR+CV
+        const R m0 = vrcv ? co.m0 : v0, m1 = vrcv ? co.m1 : v1;
+        R dy = co.c * Yold * h + (vrcv ? Yold * (m0 * i0 + m1 * i1) : dxi);
         if constexpr (LT) dy = dy + ly;
         s.Y = Yold + dy;
-        b0 = b0 - (co.s00 * v0 + co.s01 * v1);
-        b1 = b1 - (co.s10 * v0 + co.s11 * v1);
+        b0 = b0 - (co.s00 * m0 + co.s01 * m1);
+        b1 = b1 - (co.s10 * m0 + co.s11 * m1);
     } else {
         R dy = co.c * Yold * h;
         if constexpr (LT) dy = dy + ly;
@@ -538,8 +545,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
 // ----------------------------------------------------------------------------
 // FP64 control-variate plain kernels fit three waves per SIMD without spilling (on cfg2 that runs
 // them 23 % faster than the compiler's own two-wave allocation); the run-time family variants
-// would spill there and keep the default
-template <class R, int FAM> constexpr int plain_waves() { return sizeof(R) == 8 && FAM == STEP_CV ? 3 : 1; }
+// would spill there and take two.  No kernel is given a one-wave budget: above 256 registers the
+// compiler spills into accumulation registers, and the reflecting FP64 plain kernel built so
+// (264 registers, 8 of them accumulation registers) returned wrong G rows while its two-wave build
+// (256 registers) is bit-exact with the oracle.
+template <class R, int FAM> constexpr int plain_waves() { return sizeof(R) == 8 && FAM == STEP_CV ? 3 : 2; }
 
 template <class R, int PDE, int GEOM, int FAM, bool REFL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_waves<R, FAM>()))) plain_kernel(PathKernelArgs a) {
@@ -697,7 +707,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);
 // ----------------------------------------------------------------------------
 // REFL: the sides of a.rect in the mask a.pp.refl reflect (as in Walker::step).
 template <class R, int PDE, int FAM, bool REFL = false>
-__global__ void __launch_bounds__(256) paths_kernel(PathsArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) paths_kernel(PathsArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const R h = (R)a.h, sqrth = (R)a.sqrth;

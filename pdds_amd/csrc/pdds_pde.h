@@ -10,21 +10,24 @@
 
 namespace pdds {
 
-enum PdeId { PDE_MONEGROS = 0, PDE_SINSIN = 1, PDE_VARCOEF = 2, PDE_VARCOEF_LUT = 3, PDE_HELMHOLTZ = 4 };
+enum PdeId { PDE_MONEGROS = 0, PDE_SINSIN = 1, PDE_VARCOEF = 2, PDE_VARCOEF_LUT = 3, PDE_HELMHOLTZ = 4, PDE_TABLE = 5 };
 // Step families, the values of pdds_run.step.  Kernels are compiled for STEP_CV alone (the
 // default step, no family test in the loop) and for STEP_ANY (the family read at run time).
-enum StepFamily { STEP_ANY = -1, STEP_CV = 0, STEP_PLAIN = 1, STEP_VR = 2 };
+enum StepFamily { STEP_ANY = -1, STEP_CV = 0, STEP_PLAIN = 1, STEP_VR = 2, STEP_VRCV = 3 };
 
 template <class R> struct Coef {
     R s00, s01, s10, s11;  // sigma, row-major
     R b0, b1;
     R c, f;
     R g0, g1;              // grad u
+    R m0, m1, F0, F1;      // STEP_VRCV: the drift change mu and the control variate's F
 };
 
 struct PdeParams {
     double k2;             // HELMHOLTZ (launches without per-knot values)
-    LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c
+    LutView lut;           // VARCOEF_LUT: sigma11, sigma22, b1, b2, c; TABLE: those, f, du/dx, du/dy
+    LutView ulut;          // TABLE: u (the Dirichlet data g, and u of the VR step)
+    LutView vlut;          // STEP_VRCV: mu1, mu2, F1, F2
     int32_t step;          // StepFamily: PLAIN takes grad u as 0, VR hands out grad u / u
     int32_t refl;          // bit s: global side s (0 N, 1 S, 2 E, 3 W) reflects (Neumann / Robin)
     double robin;          // varphi on reflecting sides (<= 0); psi = grad u . N - varphi u
@@ -84,6 +87,15 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Co
             R uu = sx * sy + x * x + y * y;
             co.c = -k2; co.f = flap + k2 * uu;
         }
+    } else if constexpr (PDE == PDE_TABLE) {
+        // every coefficient tabulated (the reference's texMode 4 serves f, c and grad u from
+        // textures, Solvers/MCinCUDA.cu:453-478; here sigma and b as well), so a problem runs
+        // from its samples alone
+        R v[8];
+        lut_eval<R, 8>(P.lut, x, y, v);
+        co.s00 = v[0]; co.s01 = 0; co.s10 = 0; co.s11 = v[1];
+        co.b0 = v[2]; co.b1 = v[3]; co.c = v[4]; co.f = v[5];
+        co.g0 = v[6]; co.g1 = v[7];
     } else {  // VARCOEF, VARCOEF_LUT
         R sxn, cxn, syn, cyn;
         RM<R>::sincos(x, sxn, cxn);
@@ -119,6 +131,11 @@ PDDS_HD void pde_coef_t(R x, R y, const PdeParams &P, const LutTile &T, R k2, Co
             const R uu = pde_g<PDE, R>(x, y, P);     // u (every shipped problem has g = u)
             co.g0 = co.g0 / uu;
             co.g1 = co.g1 / uu;
+        } else if (P.step == STEP_VRCV) {
+            // mu and F from their table (Integrators/FKACIntegrator.hpp:100-112)
+            R w[4];
+            lut_eval<R, 4>(P.vlut, x, y, w);
+            co.m0 = w[0]; co.m1 = w[1]; co.F0 = w[2]; co.F1 = w[3];
         }
     }
 }
@@ -146,6 +163,11 @@ PDDS_HD R pde_psi(R x, R y, R nx, R ny, R varphi, const PdeParams &P) {
         RM<R>::sincospi(y, sy, cy);
         gx = (R)3.141592653589793 * cx * sy + (R)2.0 * x;
         gy = (R)3.141592653589793 * sx * cy + (R)2.0 * y;
+    } else if constexpr (PDE == PDE_TABLE) {
+        R v[8];
+        lut_eval<R, 8>(P.lut, x, y, v);
+        gx = v[6];
+        gy = v[7];
     } else {
         R s, c;
         RM<R>::sincos(y, s, c);
@@ -156,11 +178,16 @@ PDDS_HD R pde_psi(R x, R y, R nx, R ny, R varphi, const PdeParams &P) {
     return (gx * nx + gy * ny) - varphi * pde_g<PDE, R>(x, y, P);
 }
 
-// Dirichlet data g = u (all shipped problems are manufactured solutions)
+// Dirichlet data g = u (the compiled-in problems are manufactured solutions; TABLE reads u from
+// its one-field table)
 template <int PDE, class R>
 PDDS_HD R pde_g(R x, R y, const PdeParams &P) {
     (void)P;
-    if constexpr (PDE == PDE_MONEGROS) {
+    if constexpr (PDE == PDE_TABLE) {
+        R u[1];
+        lut_eval<R, 1>(P.ulut, x, y, u);
+        return u[0];
+    } else if constexpr (PDE == PDE_MONEGROS) {
         R u, gx, gy, lap;
         monegros<R>(x, y, u, gx, gy, lap);
         return u;

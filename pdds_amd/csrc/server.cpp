@@ -115,7 +115,7 @@ void prepare(pdds_server *s, SrvReq *r) {
             r->G.assign(std::max<int64_t>(M.row_ptr.back(), 1), 0.0);
             r->B.assign(n, 0.0);
             for (int q = 0; q < n; ++q) {
-                if (M.on_boundary[q]) r->B[q] = host_g(s->cfg.problem.pde, M.x0[q], M.y0[q], s->pp);
+                if (M.on_boundary[q]) r->B[q] = host_g(&s->cfg.problem, M.x0[q], M.y0[q], s->pp);
                 else r->mc.push_back(q);
             }
             r->res.n_knots = n;
@@ -409,7 +409,7 @@ pdds_status pdds_server_submit(pdds_server *s, const pdds_request *rq, uint64_t 
         const double dist = rect_dist<double>(d, rq->x, rq->y, ex, ey, nx, ny, kd);
         if (std::fabs(dist) <= 1e-8 || dist > 0.0) {
             // on the boundary: the Dirichlet value, no Monte Carlo
-            rp->res.u = host_g(s->cfg.problem.pde, ex, ey, s->pp);
+            rp->res.u = host_g(&s->cfg.problem, ex, ey, s->pp);
             rp->res.u_var = 0.0;
             finish(s, rp, PDDS_OK);
             return PDDS_OK;

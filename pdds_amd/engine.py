@@ -13,7 +13,7 @@ from . import _capi as A
 from ._capi import PddsError, ptr, f64
 
 _PDE_NAMES = {"monegros": A.PDE_MONEGROS, "sinsin": A.PDE_SINSIN, "varcoef": A.PDE_VARCOEF,
-              "varcoef_lut": A.PDE_VARCOEF_LUT, "helmholtz": A.PDE_HELMHOLTZ}
+              "varcoef_lut": A.PDE_VARCOEF_LUT, "helmholtz": A.PDE_HELMHOLTZ, "table": A.PDE_TABLE}
 
 
 class bvp:
@@ -23,23 +23,31 @@ class bvp:
     `u(x, y)` evaluates the manufactured solution on the host (for error reports).
     """
 
-    def __init__(self, pde="monegros", k2=0.0, lut=None, neumann=0, varphi=0.0, rho=0.0):
+    def __init__(self, pde="monegros", k2=0.0, lut=None, neumann=0, varphi=0.0, rho=0.0, ulut=None, vlut=None):
         """neumann: reflecting global sides (bits SIDE_N/S/E/W of _capi), where du/dn = varphi u + psi
-        (varphi <= 0; psi from the solution); rho: Lepingle step within this distance of them."""
+        (varphi <= 0; psi from the solution); rho: Lepingle step within this distance of them.
+        "table": every coefficient from `lut` (8 fields: sigma11, sigma22, b1, b2, c, f, du/dx,
+        du/dy) and u = g from `ulut` (1 field).  vlut: mu1, mu2, F1, F2 of the VR+CV step."""
         self.pde = _PDE_NAMES[pde] if isinstance(pde, str) else int(pde)
         self.k2 = float(k2)
         self.lut = lut
+        self.ulut = ulut
+        self.vlut = vlut
         self.neumann = int(neumann)
         self.varphi = float(varphi)
         self.rho = float(rho)
         if self.pde == A.PDE_VARCOEF_LUT and lut is None:
             raise ValueError("varcoef_lut needs a LUT")
+        if self.pde == A.PDE_TABLE and (lut is None or ulut is None):
+            raise ValueError("table needs lut and ulut")
 
     def _c(self):
         p = A.Problem()
         p.pde = self.pde
         p.k2 = self.k2
         p.lut = self.lut.h if self.lut is not None else None
+        p.ulut = self.ulut.h if self.ulut is not None else None
+        p.vlut = self.vlut.h if self.vlut is not None else None
         p.refl = self.neumann
         p.robin = self.varphi
         p.rho = self.rho
@@ -48,6 +56,10 @@ class bvp:
     def u(self, x, y):
         x = np.asarray(x, dtype=np.float64)
         y = np.asarray(y, dtype=np.float64)
+        if self.pde == A.PDE_TABLE:
+            if not hasattr(self, "u_exact"):
+                raise ValueError("a tabulated problem has no closed-form u; set bvp.u_exact")
+            return self.u_exact(x, y)
         if self.pde == A.PDE_MONEGROS:
             k1, k2, k3, k4, k5, k6, k7, sf = 1.0, 0.01, 0.02, 0.12, 0.05, 0.05, -0.12, 0.333
             return 3.0 + sf * (np.sin(np.sqrt(k1 + k2 * x * x + k3 * y * y))

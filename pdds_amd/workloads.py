@@ -39,6 +39,51 @@ def varcoef_lut(engine, n=512, box=(-1.0, -1.0, 1.0, 1.0), mode=0):
     return engine.lut(varcoef_fields(X, Y), box, mode=mode)
 
 
+def varcoef_u(x, y):
+    return np.exp(x) * np.cos(y)
+
+
+def varcoef_table_fields(x, y):
+    """The same PDE as eight tabulated fields -- sigma11, sigma22, b1, b2, c, f, du/dx, du/dy --
+    with f from u = e^x cos y (the tabulated problem, bvp("table"))."""
+    s1 = 1.0 + 0.3 * np.sin(x)
+    s2 = 1.0 + 0.3 * np.cos(y)
+    uu = np.exp(x) * np.cos(y)
+    uy = -np.exp(x) * np.sin(y)
+    a1, a2 = s1 * s1, s2 * s2
+    a1p, a2p = 2.0 * s1 * (0.3 * np.cos(x)), 2.0 * s2 * (-0.3 * np.sin(y))
+    fcfg = 0.5 * (a1p * uu + a1 * uu + a2p * uy - a2 * uu) + 0.2 * y * uu + (-0.2 * x) * uy - (1.0 + x * x) * uu
+    f5 = varcoef_fields(x, y)
+    return np.concatenate([f5, np.stack([-fcfg, uu, uy], axis=-1)], axis=-1)
+
+
+def varcoef_vrcv_fields(x, y, theta):
+    """mu and F of the VR+CV step for the same PDE: mu = theta (-sigma^T grad u / u) and
+    F = -(1 - theta) sigma^T grad u, so that F = -sigma^T grad u - u mu (the score's martingale part
+    vanishes; theta = 1 is the VR step's drift, theta = 0 the control variate alone)."""
+    s1 = 1.0 + 0.3 * np.sin(x)
+    s2 = 1.0 + 0.3 * np.cos(y)
+    uu = np.exp(x) * np.cos(y)
+    ux, uy = uu, -np.exp(x) * np.sin(y)
+    v1, v2 = s1 * ux, s2 * uy
+    return np.stack([-theta * v1 / uu, -theta * v2 / uu, -(1.0 - theta) * v1, -(1.0 - theta) * v2], axis=-1)
+
+
+def table_problem(engine, n=128, box=(-1.0, -1.0, 1.0, 1.0), theta=None):
+    """bvp("table") of the divergence-form PDE from n x n node samples on box; with theta, also the
+    VR+CV table (mu, F) at that theta."""
+    from .engine import bvp
+    xs = np.linspace(box[0], box[2], n)
+    ys = np.linspace(box[1], box[3], n)
+    X, Y = np.meshgrid(xs, ys)
+    lut = engine.lut(varcoef_table_fields(X, Y), box)
+    ulut = engine.lut(varcoef_u(X, Y), box)
+    vlut = engine.lut(varcoef_vrcv_fields(X, Y, theta), box) if theta is not None else None
+    p = bvp("table", lut=lut, ulut=ulut, vlut=vlut)
+    p.u_exact = varcoef_u
+    return p
+
+
 CONFIGS = {
     # name: (sw, ne, n_sub, n_node, pde, paths per knot, h, seed, shape)
     # cfg2's 94-knot stencil sides are singular at shape 1 (cond > 1/eps); 0.8 keeps every side

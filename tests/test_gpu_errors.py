@@ -36,7 +36,7 @@ def _status(fn):
 
 
 @pytest.mark.parametrize("field,value,msg", [
-    ("step", 3, "step family"),
+    ("step", 4, "step family"),
     ("step", -1, "step family"),
     ("max_steps", 1 << 27, "max_steps"),
     ("precision", 2, "precision"),
@@ -73,6 +73,18 @@ def test_bad_counts_and_problems(eng, mesh):
     assert eng.paths(bvp("sinsin"), run, 0.1, 0.1, rect, 0, 0, 0)["counters"]["paths"] == 0
     st, _ = _status(lambda: eng.paths(bvp("sinsin"), run, 0.1, 0.1, rect, 0, (1 << 32) - 1, 2))
     assert st == A.ERR_INVALID
+    # tables of the wrong shape: a tabulated problem needs 8 + 1 fields, VR+CV a 4-field table
+    xs = np.linspace(-1, 1, 8)
+    X, Y = np.meshgrid(xs, xs)
+    l5 = eng.lut(np.stack([X] * 5, axis=-1), rect)
+    l1 = eng.lut(X, rect)
+    st, text = _status(lambda: eng.paths(bvp("table", lut=l5, ulut=l1), run, 0.1, 0.1, rect, 0, 0, 8))
+    assert st == A.ERR_INVALID and "8-field" in text
+    st, text = _status(lambda: eng.paths(bvp("sinsin"), default_run(h=1e-3, seed=1, step=3), 0.1, 0.1, rect, 0, 0, 8))
+    assert st == A.ERR_INVALID and "vlut" in text
+    st, text = _status(lambda: eng.paths(bvp("sinsin", vlut=l5), default_run(h=1e-3, seed=1, step=3), 0.1, 0.1, rect,
+                                         0, 0, 8))
+    assert st == A.ERR_INVALID and "vlut" in text
 
 
 def test_engine_usable_after_errors(eng, mesh):

@@ -128,6 +128,12 @@ typedef struct {
     int32_t paths_per_lane; /* persistent launches: a wave takes 64 x this many paths from its queue
                                at once (0: 4); results do not depend on it */
     int32_t step;           /* step family (PDDS_STEP_*, default 0) */
+    int32_t adaptive_h;     /* 1: each knot's paths step with h_k = min(h, (d / (2 |beta| |N^T sigma(X0)|))^2),
+                               d the knot's signed distance to its stopping domain and N that side's
+                               normal, when d <= -1e-4 (else h): the Gobet shift is at most half the
+                               distance to the nearest side (PDDAlgorithms/PDDSparseGM.cpp:4025-4029,
+                               :4386-4389).  0: every path steps with h (default).  Not with
+                               reflecting sides. */
 } pdds_run;
 void pdds_run_defaults(pdds_run *r);
 
@@ -210,6 +216,11 @@ pdds_status pdds_paths(pdds_engine *e, const pdds_problem *p, const pdds_run *r,
 pdds_status pdds_solve_knot(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r,
                             int32_t knot, int64_t n_paths, double *G, double *G_var, double *B, double *BB,
                             pdds_counters *ctr);
+
+/* ---- the step each knot's paths take under r (r->h, or the adaptive_h rule), h[n_knots], 0 for
+   boundary knots: the h column of the reference's Compute_h_N output
+   (PDDAlgorithms/PDDSparseGM.cpp:4386-4390) ---- */
+pdds_status pdds_knot_steps(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r, double *h);
 
 /* ---- every knot of a mesh: G values in the mesh CSR pattern, per-entry per-path variance,
    B and B second moment per knot.  paths_per_knot may be NULL (n_paths each, n_paths >= 1).

@@ -34,7 +34,7 @@ class Run(C.Structure):
                 ("precision", C.c_int32), ("quirk_last_side", C.c_int32), ("max_steps", C.c_int64),
                 ("sub_batch", C.c_int32), ("item_paths", C.c_int32), ("wg_size", C.c_int32),
                 ("grid_wgs", C.c_int32), ("lut_lds", C.c_int32), ("plain", C.c_int32),
-                ("paths_per_lane", C.c_int32), ("step", C.c_int32)]
+                ("paths_per_lane", C.c_int32), ("step", C.c_int32), ("adaptive_h", C.c_int32)]
 
 
 class Adaptive(C.Structure):
@@ -131,6 +131,7 @@ EXPORTS = {
                                _d, _d, _d, _d, _d, _i64, C.POINTER(Counters)]),
     "pdds_solve_knot": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), C.c_int32,
                                     C.c_int64, _d, _d, _d, _d, C.POINTER(Counters)]),
+    "pdds_knot_steps": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _d]),
     "pdds_assemble": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _i64, C.c_int64,
                                   _d, _d, _d, _d, C.POINTER(Counters)]),
     "pdds_assemble_eq": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Problem), C.POINTER(Run), _i64, C.c_int64,

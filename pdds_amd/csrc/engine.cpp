@@ -135,6 +135,42 @@ double host_g(const pdds_problem *p, double x, double y, const PdeParams &pp) {
     }
 }
 
+// sigma at (x, y) on the host (tabulated coefficients from the host copies of their tables)
+static void host_sigma(const pdds_problem *p, const PdeParams &pp, double x, double y, double s[4]) {
+    PdeParams hp = pp;
+    if (p->lut) hp.lut = p->lut->hview;
+    if (p->ulut) hp.ulut = p->ulut->hview;
+    Coef<double> co;
+    switch (p->pde) {
+        case PDE_MONEGROS: pde_coef<PDE_MONEGROS, double>(x, y, hp, co); break;
+        case PDE_SINSIN: pde_coef<PDE_SINSIN, double>(x, y, hp, co); break;
+        case PDE_VARCOEF: pde_coef<PDE_VARCOEF, double>(x, y, hp, co); break;
+        case PDE_VARCOEF_LUT: pde_coef<PDE_VARCOEF_LUT, double>(x, y, hp, co); break;
+        case PDE_HELMHOLTZ: pde_coef<PDE_HELMHOLTZ, double>(x, y, hp, co); break;
+        default: pde_coef<PDE_TABLE, double>(x, y, hp, co); break;
+    }
+    s[0] = co.s00; s[1] = co.s01; s[2] = co.s10; s[3] = co.s11;
+}
+
+// Step of a knot's paths: the run's h, or with adaptive_h the distance rule of
+// PDDAlgorithms/PDDSparseGM.cpp:4025-4029 -- h_k = min(h, (d / (2 |beta| |N^T sigma(X0)|))^2) for
+// a start point at signed distance d <= -1e-4 from its stopping domain (N the nearest side's
+// outward normal), so that the Gobet shift |beta| |N^T sigma| sqrt(h_k) is at most d / 2
+double knot_step(const pdds_problem *p, const PdeParams &pp, const pdds_run *r, int geom, const double rect[4],
+                 const double geo[6], double x0, double y0) {
+    if (!r->adaptive_h) return r->h;
+    double ex, ey, nx, ny;
+    int kind;
+    const double d = geom == GeomDisc ? sector_dist<double>(geo, x0, y0, ex, ey, nx, ny, kind)
+                                      : rect_dist<double>(rect, x0, y0, ex, ey, nx, ny, kind);
+    if (!(d <= -1e-4)) return r->h;
+    double s[4];
+    host_sigma(p, pp, x0, y0, s);
+    const double ga = nx * s[0] + ny * s[2], gb = nx * s[1] + ny * s[3];
+    const double q = d / (std::sqrt(ga * ga + gb * gb) * 2.0 * std::fabs(r->beta));
+    return std::min(r->h, q * q);
+}
+
 pdds_status check_run(pdds_engine *e, const pdds_run *r) {
     if (!r) return fail(e, PDDS_ERR_INVALID, "null run parameters");
     if (!(r->h > 0.0) || !std::isfinite(r->h)) return fail(e, PDDS_ERR_INVALID, "h must be positive");
@@ -149,6 +185,7 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
                       PDDS_STEP_VRCV == STEP_VRCV && PDDS_PDE_TABLE == PDE_TABLE,
                   "step family and problem values");
     if (r->step < PDDS_STEP_CV || r->step > PDDS_STEP_VRCV) return fail(e, PDDS_ERR_INVALID, "bad step family");
+    if (r->adaptive_h != 0 && r->adaptive_h != 1) return fail(e, PDDS_ERR_INVALID, "adaptive_h must be 0 or 1");
     return PDDS_OK;
 }
 
@@ -302,6 +339,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             kd[q].table = k.table; kd[q].stream = k.has_stream ? k.stream : r->stream;
             kd[q].n_paths = k.n; kd[q].path_off = off; kd[q].out_off = k.out_off;
             kd[q].k2 = k.has_k2 ? k.k2 : pp.k2;
+            kd[q].h = knot_step(prob, pp, r, geom, sd.rect, sd.geo, k.x0, k.y0);
+            kd[q].sqrth = std::sqrt(kd[q].h);
             kd[q].item_begin = (int64_t)items.size();
             for (int64_t p0 = 0; p0 < k.n; p0 += P) {
                 ItemDesc it;
@@ -316,11 +355,12 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             off += k.n;
         }
         items_all += (int64_t)items.size();
-        // knots of other streams or k2 than the run's (a serving round): per-lane values
+        // knots of other streams, k2 or h than the run's (a serving round, adaptive h): per-lane values
         bool mixed = false;
-        for (int q = 0; q < nk; ++q) mixed = mixed || kd[q].stream != r->stream || kd[q].k2 != pp.k2;
+        for (int q = 0; q < nk; ++q)
+            mixed = mixed || kd[q].stream != r->stream || kd[q].k2 != pp.k2 || kd[q].h != r->h;
         if (mixed && pp.refl)
-            return fail(e, PDDS_ERR_UNSUPPORTED, "reflecting sides with per-knot streams or k2 in one launch");
+            return fail(e, PDDS_ERR_UNSUPPORTED, "reflecting sides with per-knot streams, k2 or h in one launch");
         const size_t shmem = reduce_lds_bytes(ne_cap, nnz_cap, S);
         if (shmem > 160 * 1024 || S > 65535)
             return fail(e, PDDS_ERR_UNSUPPORTED, "stencil or sub-batch too large for the LDS reduction buffers");
@@ -1190,6 +1230,21 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         rep->best_rate = best_rate;
         rep->base_rate = base_rate;
         rep->total_ms = now_ms() - t0;
+    }
+    return PDDS_OK;
+}
+
+pdds_status pdds_knot_steps(pdds_engine *e, const pdds_mesh *m, const pdds_problem *p, const pdds_run *r, double *h) {
+    if (!e || !m || !h) return PDDS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(e->mu);
+    pdds_status st = check_run(e, r);
+    if (st) return st;
+    PdeParams pp;
+    if ((st = make_params(e, p, r, pp))) return st;
+    const Mesh &M = m->m;
+    for (int q = 0; q < M.n_knots(); ++q) {
+        const Stencil &S = M.stencils[M.stencil_of[q]];
+        h[q] = M.on_boundary[q] ? 0.0 : knot_step(p, pp, r, M.geom, S.rect, S.geo, M.x0[q], M.y0[q]);
     }
     return PDDS_OK;
 }

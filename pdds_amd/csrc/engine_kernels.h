@@ -313,8 +313,9 @@ template <int GEOM> constexpr int domain_len() { return GEOM == GeomDisc ? 6 : 4
 // ----------------------------------------------------------------------------
 // Per-lane walker: the state of the path a lane carries and one step of it
 // ----------------------------------------------------------------------------
-// MIXED: the knots of the launch differ in Philox stream or Helmholtz k2 (serving rounds), so
-// each lane carries its knot's; otherwise both are the launch's and stay in scalar registers.
+// MIXED: the knots of the launch differ in Philox stream, Helmholtz k2 (serving rounds) or step h
+// (adaptive h), so each lane carries its knot's; otherwise they are the launch's and stay in
+// scalar registers.
 // REFL: some global sides reflect (rectangles only): the path carries its nearest side (distance,
 // kind, outward normal) and the local time of a projection, and steps with refl_step while that
 // side reflects; only absorbing sides stop it.
@@ -329,6 +330,7 @@ struct Walker {
     uint32_t g;            // batch path index (record slot; a batch holds fewer than 2^32 paths)
     uint32_t kstream;      // Philox key word 1 of the path's knot
     R k2;                  // Helmholtz parameter of the path's knot
+    R hk, sqhk;            // step of the path's knot and its square root
     bool active, fresh;
     // REFL: reflecting sides of the stopping domain, nearest side at the current X, pending
     // local time
@@ -336,7 +338,7 @@ struct Walker {
     R rd, rnx, rny, lt;
 
     __device__ __forceinline__ void init() {
-        pn1 = 0; pn2 = 0; kid = 0; kstream = 0; gmask = 0; g = 0; k2 = 0;
+        pn1 = 0; pn2 = 0; kid = 0; kstream = 0; gmask = 0; g = 0; k2 = 0; hk = 0; sqhk = 0;
         active = false; fresh = false;
         rmask = 0; rk = 0; rd = 0; rnx = 0; rny = 0; lt = 0;
     }
@@ -352,6 +354,8 @@ struct Walker {
         if constexpr (MIXED) {
             kstream = kd.stream;
             k2 = (R)kd.k2;
+            hk = (R)kd.h;
+            sqhk = (R)kd.sqrth;
         }
         if constexpr (REFL) {
             rmask = sd.gmask & a.pp.refl;   // stencil sides on reflecting global sides
@@ -376,7 +380,7 @@ struct Walker {
     // it: the wave's iteration count (stepping lanes have s.st of its parity)
     template <bool TILED, bool DRAW_FIRST>
     __device__ __forceinline__ void step(const PathKernelArgs &a, const LutTile &T, int it) {
-        const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta;
+        const R h = MIXED ? hk : (R)a.h, sqrth = MIXED ? sqhk : (R)a.sqrth, beta = (R)a.beta;
         const uint32_t strm = MIXED ? kstream : a.stream;
         const R kk2 = MIXED ? k2 : (R)a.pp.k2;
         const bool stepping = active && !fresh;
@@ -558,7 +562,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
     const KnotDesc &kd = a.knots[find_knot(a.knots, a.n_knots, gg)];
     const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
     const LutTile none{nullptr, 0, 0, 0, 0};
-    const R h = (R)a.h, sqrth = (R)a.sqrth, beta = (R)a.beta, k2 = (R)kd.k2;
+    const R h = (R)kd.h, sqrth = (R)kd.sqrth, beta = (R)a.beta, k2 = (R)kd.k2;
     R rect[domain_len<GEOM>()];
     const double *dom = domain_of<GEOM>(sd);
     for (int q = 0; q < domain_len<GEOM>(); ++q) rect[q] = (R)dom[q];

@@ -38,6 +38,7 @@ struct KnotDesc {
     int64_t item_begin;     // first work item of the knot
     int64_t item_count;
     double k2;              // Helmholtz parameter of the knot's problem
+    double h, sqrth;        // Euler-Maruyama step of the knot's paths (the run's, or its adaptive h)
 };
 
 // One distinct stencil: stopping rectangle, four side lists (N,S,E,W), Psi^-1 per side,

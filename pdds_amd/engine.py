@@ -144,6 +144,12 @@ class Engine:
         return {"cols": mesh.row_cols(knot), "G": G[:nnz], "Gvar": Gv[:nnz], "B": B.value, "BB": BB.value,
                 "counters": ctr.as_dict()}
 
+    def knot_steps(self, mesh, problem, run):
+        """Step h of each knot's paths under run (its adaptive_h rule); 0 for boundary knots."""
+        h = np.zeros(mesh.n_knots)
+        self._chk(A.lib().pdds_knot_steps(self.h, mesh.h, C.byref(problem._c()), C.byref(run), ptr(h)), "knot_steps")
+        return h
+
     def assemble(self, mesh, problem, run, n_paths, paths_per_knot=None):
         nnz = mesh.nnz
         G = np.zeros(max(nnz, 1))

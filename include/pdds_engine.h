@@ -347,9 +347,11 @@ pdds_status pdds_solve(pdds_engine *e, int32_t n, const int64_t *row_ptr, const 
    one device thread then runs rounds: each round admits prepared requests in arrival order
    while their device footprint fits hbm_budget and fewer than max_inflight are admitted,
    packs the not-yet-run work of every admitted request (point paths first, then solve knots,
-   at most round_paths paths) into one persistent launch, and completes every request whose
-   work is done (a solve after its device BiCGSTAB).  A solve larger than a round is spread
-   over rounds, so requests arriving meanwhile join the next round.  The paths of a request
+   at most round_paths paths) into one persistent launch, and completes every point whose paths
+   are done; a solve whose knots are all done goes to a solver thread, which runs its device
+   BiCGSTAB on a second engine (its own stream, default priority) while the next round's paths
+   run.  A solve larger than a round is spread over rounds, so requests arriving meanwhile join
+   the next round.  The paths of a request
    use its own Philox stream, and each of its sums has a fixed order: its result is the same,
    bit for bit, alone or in any mix.  The server uses the engine's stream and lock; other
    engine calls made meanwhile wait for the current round. */
@@ -378,6 +380,9 @@ typedef struct {
     int32_t n_sub, n_node;  /* solve: subdomains per side, knots per interface */
     double fac[4];          /* solve: RBF shape per stencil kind */
     double tol;             /* solve: BiCGSTAB relative tolerance (0: 1e-10) */
+    double h;               /* Euler-Maruyama step of this request's paths (0: the server run's h); the
+                               step bias of a solve grows as its subdomains shrink, so a fixed error
+                               target asks for h in proportion to the squared subdomain width */
 } pdds_request;
 
 typedef struct {

@@ -88,7 +88,7 @@ REQ_POINT, REQ_SOLVE = 0, 1
 class Request(C.Structure):
     _fields_ = [("kind", C.c_int32), ("stream", C.c_uint32), ("k2", C.c_double), ("n_paths", C.c_int64),
                 ("x", C.c_double), ("y", C.c_double), ("n_sub", C.c_int32), ("n_node", C.c_int32),
-                ("fac", C.c_double * 4), ("tol", C.c_double)]
+                ("fac", C.c_double * 4), ("tol", C.c_double), ("h", C.c_double)]
 
 
 class Result(C.Structure):

@@ -339,7 +339,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             kd[q].table = k.table; kd[q].stream = k.has_stream ? k.stream : r->stream;
             kd[q].n_paths = k.n; kd[q].path_off = off; kd[q].out_off = k.out_off;
             kd[q].k2 = k.has_k2 ? k.k2 : pp.k2;
-            kd[q].h = knot_step(prob, pp, r, geom, sd.rect, sd.geo, k.x0, k.y0);
+            kd[q].h = k.h > 0.0 ? k.h : knot_step(prob, pp, r, geom, sd.rect, sd.geo, k.x0, k.y0);
             kd[q].sqrth = std::sqrt(kd[q].h);
             kd[q].item_begin = (int64_t)items.size();
             for (int64_t p0 = 0; p0 < k.n; p0 += P) {

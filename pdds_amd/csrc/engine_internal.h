@@ -132,6 +132,7 @@ struct AsmKnot {
     double k2 = 0.0;
     bool has_stream = false;// per-knot Philox stream (serving), else the run's
     uint32_t stream = 0;
+    double h = 0.0;         // per-knot step (serving), else 0: the run's h or its adaptive_h rule
 };
 
 struct AsmOut {

@@ -81,10 +81,18 @@ struct pdds_server {
     std::map<uint64_t, std::unique_ptr<SrvReq>> reqs;
     std::vector<std::thread> preps;
     std::thread dev;
+    // interface solves run on a second engine (its own stream and buffers) in their own thread,
+    // so that the next round's paths run on the device while a finished request's BiCGSTAB does
+    pdds_engine *se = nullptr;
+    std::thread solver;
+    std::deque<SrvReq *> to_solve;
+    std::condition_variable cv_solve;
+    bool dev_done = false;
     bool stop = false;
     uint64_t next_ticket = 1;
     int64_t inflight_bytes = 0;
     int64_t n_submitted = 0, n_done = 0, n_rounds = 0, n_steps = 0;
+    int64_t n_solver = 0;             // requests handed to the solver thread and not finished
     double busy_ms = 0;
     DevTables dom;                    // the domain's one-stencil table (point requests)
     bool dom_ready = false;
@@ -209,7 +217,7 @@ bool run_round(pdds_server *s) {
         if (r->rq.kind != PDDS_REQ_POINT || r->ran || !s->dom_ready) continue;
         if (!knots.empty() && paths + r->rq.n_paths > cap) continue;
         AsmKnot k{r->rq.x, r->rq.y, 0u, 0, r->rq.n_paths, 0};
-        k.has_k2 = true; k.k2 = r->rq.k2; k.has_stream = true; k.stream = r->rq.stream;
+        k.has_k2 = true; k.k2 = r->rq.k2; k.has_stream = true; k.stream = r->rq.stream; k.h = r->rq.h;
         knots.push_back(k);
         who.push_back({r, -1});
         paths += r->rq.n_paths;
@@ -223,7 +231,7 @@ bool run_round(pdds_server *s) {
             if (ti < 0) { ti = (int)tabs.size(); tabs.push_back(&r->tab); }
             const int q = r->mc[r->next++];
             AsmKnot k{M.x0[q], M.y0[q], (uint32_t)q, M.stencil_of[q], r->rq.n_paths, n_vals};
-            k.table = ti; k.has_k2 = true; k.k2 = r->rq.k2; k.has_stream = true; k.stream = r->rq.stream;
+            k.table = ti; k.has_k2 = true; k.k2 = r->rq.k2; k.has_stream = true; k.stream = r->rq.stream; k.h = r->rq.h;
             knots.push_back(k);
             who.push_back({r, q});
             n_vals += M.row_ptr[q + 1] - M.row_ptr[q];
@@ -253,28 +261,15 @@ bool run_round(pdds_server *s) {
             r->finished += 1;
         }
     }
-    // solves whose knots are all done
-    std::vector<SrvReq *> complete;
+    // requests whose paths are all done: points complete here, solves go to the solver thread
+    std::vector<SrvReq *> complete, solve;
     for (SrvReq *r : adm) {
-        bool done = false;
         pdds_status rs = st;
+        bool done = false;
         if (std::find(failed.begin(), failed.end(), r) != failed.end()) { done = true; rs = PDDS_ERR_NOMEM; }
         else if (st) done = true;
         else if (r->rq.kind == PDDS_REQ_POINT) done = r->ran;
-        else if (r->finished == r->mc.size()) {
-            const Mesh &M = r->mesh->m;
-            const int n = M.n_knots();
-            r->u.assign(n, 0.0);
-            int32_t it = 0;
-            double rr = 0;
-            const double tol = r->rq.tol > 0 ? r->rq.tol : 1e-10;
-            std::vector<int32_t> cols(M.cols.begin(), M.cols.end());
-            rs = solve_system(e, n, M.row_ptr.data(), cols.data(), r->G.data(), r->B.data(), r->u.data(), tol, 0,
-                              &it, &rr);
-            r->res.iters = it;
-            r->res.relres = rr;
-            done = true;
-        }
+        else if (r->finished == r->mc.size()) solve.push_back(r);
         if (done) {
             r->res.status = rs;
             complete.push_back(r);
@@ -285,14 +280,56 @@ bool run_round(pdds_server *s) {
     s->busy_ms += dt;
     s->n_steps += steps;
     ++s->n_rounds;
-    for (SrvReq *r : complete) {
+    auto release = [&](SrvReq *r) {
         r->tab.release();
         r->sten.clear(); r->entx.clear(); r->enty.clear(); r->ipsi.clear(); r->entic2.clear(); r->ents.clear(); r->e1.clear(); r->e2.clear();
         s->inflight_bytes -= r->footprint;
         s->admitted.erase(std::find(s->admitted.begin(), s->admitted.end(), r));
+    };
+    for (SrvReq *r : complete) {
+        release(r);
         finish(s, r, r->res.status);
     }
+    for (SrvReq *r : solve) {
+        release(r);
+        s->to_solve.push_back(r);
+        ++s->n_solver;
+    }
+    if (!solve.empty()) s->cv_solve.notify_all();
     return true;
+}
+
+// (I + G) u = B of the solves whose knots are done, on the solver engine, in hand-over order
+void solver_loop(pdds_server *s) {
+    while (true) {
+        SrvReq *r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_solve.wait(lk, [&] { return !s->to_solve.empty() || s->dev_done; });
+            if (s->to_solve.empty()) return;
+            r = s->to_solve.front();
+            s->to_solve.pop_front();
+        }
+        const Mesh &M = r->mesh->m;
+        const int n = M.n_knots();
+        r->u.assign(n, 0.0);
+        int32_t it = 0;
+        double rr = 0;
+        const double tol = r->rq.tol > 0 ? r->rq.tol : 1e-10;
+        std::vector<int32_t> cols(M.cols.begin(), M.cols.end());
+        pdds_status rs;
+        {
+            std::lock_guard<std::mutex> lk(s->se->mu);
+            rs = solve_system(s->se, n, M.row_ptr.data(), cols.data(), r->G.data(), r->B.data(), r->u.data(), tol, 0,
+                              &it, &rr);
+        }
+        r->res.iters = it;
+        r->res.relres = rr;
+        std::lock_guard<std::mutex> lk(s->mu);
+        --s->n_solver;
+        finish(s, r, rs);
+        s->cv_work.notify_all();
+    }
 }
 
 void dev_loop(pdds_server *s) {
@@ -300,7 +337,8 @@ void dev_loop(pdds_server *s) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
-                return !s->ready.empty() || !s->admitted.empty() || (s->stop && s->prep_q.empty() && s->n_done == s->n_submitted);
+                return !s->ready.empty() || !s->admitted.empty() ||
+                       (s->stop && s->prep_q.empty() && s->n_done + s->n_solver == s->n_submitted);
             });
             if (s->ready.empty() && s->admitted.empty()) return;
         }
@@ -354,9 +392,12 @@ pdds_status pdds_server_create(pdds_engine *e, const pdds_server_config *cfg, pd
         // reflecting-side kernels do not take
         if (s->pp.refl) return fail(e, PDDS_ERR_UNSUPPORTED, "serving with reflecting sides");
     }
+    pdds_status st = pdds_engine_create(e->device, &s->se);
+    if (st) return fail(e, st, "server: solver engine");
     pdds_server *sp = s.get();
     for (int i = 0; i < c.prep_threads; ++i) sp->preps.emplace_back(prep_loop, sp);
     sp->dev = std::thread(dev_loop, sp);
+    sp->solver = std::thread(solver_loop, sp);
     *out = s.release();
     return PDDS_OK;
 }
@@ -372,17 +413,24 @@ void pdds_server_destroy(pdds_server *s) {
     s->cv_work.notify_all();
     s->dev.join();
     {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->dev_done = true;
+    }
+    s->cv_solve.notify_all();
+    s->solver.join();
+    {
         std::lock_guard<std::mutex> lk(s->e->mu);
         s->dom.release();
         for (auto &kv : s->reqs) kv.second->tab.release();
     }
+    pdds_engine_destroy(s->se);
     delete s;
 }
 
 pdds_status pdds_server_submit(pdds_server *s, const pdds_request *rq, uint64_t *ticket) {
     if (!s || !rq || !ticket) return PDDS_ERR_INVALID;
     if (rq->kind != PDDS_REQ_POINT && rq->kind != PDDS_REQ_SOLVE) return PDDS_ERR_INVALID;
-    if (rq->n_paths <= 0) return PDDS_ERR_INVALID;
+    if (rq->n_paths <= 0 || !(rq->h >= 0.0) || !std::isfinite(rq->h)) return PDDS_ERR_INVALID;
     const double *d = s->cfg.domain;
     std::unique_ptr<SrvReq> r(new SrvReq());
     r->rq = *rq;

@@ -371,14 +371,16 @@ class Server:
     def __del__(self):
         self.close()
 
-    def submit_point(self, x, y, n_paths, k2=0.0, stream=1):
+    def submit_point(self, x, y, n_paths, k2=0.0, stream=1, h=0.0):
+        """h: this request's step (0: the server run's)."""
         r = A.Request()
-        r.kind, r.stream, r.k2, r.n_paths, r.x, r.y = A.REQ_POINT, stream, k2, n_paths, x, y
+        r.kind, r.stream, r.k2, r.n_paths, r.x, r.y, r.h = A.REQ_POINT, stream, k2, n_paths, x, y, h
         return self._submit(r)
 
-    def submit_solve(self, n_sub, n_node, fac, n_paths, k2=0.0, stream=1, tol=1e-10):
+    def submit_solve(self, n_sub, n_node, fac, n_paths, k2=0.0, stream=1, tol=1e-10, h=0.0):
         r = A.Request()
         r.kind, r.stream, r.k2, r.n_paths, r.n_sub, r.n_node, r.tol = A.REQ_SOLVE, stream, k2, n_paths, n_sub, n_node, tol
+        r.h = h
         for i in range(4):
             r.fac[i] = fac[i]
         return self._submit(r)

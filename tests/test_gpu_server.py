@@ -111,6 +111,29 @@ def test_solve_matches_direct_assembly_and_solve(eng):
     assert np.sqrt(np.mean((u - ex) ** 2)) < 0.08
 
 
+def test_request_step_matches_direct_calls(eng):
+    """A request's own h: its point equals pdds_point_eval at that h, its solve pdds_assemble +
+    pdds_solve at that h, bit for bit, with other requests (the server's h) in the same rounds."""
+    from pdds_amd.engine import Mesh, bvp, default_run
+    srv = _server(eng, precision="fp64")
+    fac = _fac(3, 6)
+    tp = srv.submit_point(0.2, 0.45, 6000, k2=3.0, stream=71, h=5e-4)
+    ts = srv.submit_solve(3, 6, fac, 1200, k2=1.5, stream=72, tol=1e-12, h=7e-4)
+    mix = _mix(srv)
+    got_p, got_s = srv.wait(tp), srv.wait(ts)
+    for t in mix:
+        assert srv.wait(t)["status"] == 0
+    srv.close()
+    ref_p = eng.point_eval(bvp("helmholtz", k2=3.0), default_run(h=5e-4, seed=23, precision="fp64", stream=71),
+                           DOMAIN, [0.2], [0.45], 6000)
+    assert got_p["u"] == ref_p["u"][0] and got_p["u_var"] == ref_p["u_var"][0]
+    m = Mesh(DOMAIN[:2], DOMAIN[2:], 3, 6, fac)
+    out = eng.assemble(m, bvp("helmholtz", k2=1.5), default_run(h=7e-4, seed=23, precision="fp64", stream=72), 1200)
+    u, it, _ = eng.solve(m.n_knots, m.row_ptr, m.cols, out["G"], out["B"], tol=1e-12)
+    assert np.array_equal(got_s["u_knots"], u) and got_s["iters"] == it
+    assert got_s["path_steps"] == out["counters"]["path_steps"]
+
+
 def test_budget_and_timing_fields(eng):
     from pdds_amd._capi import ERR_FULL, PddsError
     srv = _server(eng, hbm_budget=1 << 20)

@@ -372,6 +372,12 @@ def batching_bench(eng):
     ev = eng.assemble(mesh, prob, best, n_even)
     u_even = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, ev["G"], ev["B"], tol=1e-10)[0]
     t_even = time.perf_counter() - t2
+    # the equation-variance rule at the same target (the knot equation B_k - sum_j G_kj u_j)
+    t3 = time.perf_counter()
+    eq = eng.solve_adaptive(mesh, prob, best, variance=1, **cfg)
+    wall_eq = time.perf_counter() - t3
+    req = eq["report"]
+    Neq = eq["paths_per_knot"][mc]
     return {
         "workload": "disc r=1, 6 rings x 24 sectors, 8..64 knots/interface (seed 11): %d knots (%d MC), "
                     "cfg2 PDE from 512^2 LUTs, h=%g, FP64 paths, SE target %g per knot score (the reference's "
@@ -386,6 +392,17 @@ def batching_bench(eng):
         "rms_interface_error": float(np.sqrt(np.mean(err ** 2))),
         "even_counts": {"paths_per_knot": n_even, "path_steps": ev["counters"]["path_steps"], "seconds": t_even,
                         "rms_interface_error": float(np.sqrt(np.mean((u_even - uex) ** 2)))},
+        "adaptive_over_even": {"rms": float(np.sqrt(np.mean(err ** 2))) / float(np.sqrt(np.mean((u_even - uex) ** 2))),
+                               "path_steps": rep["path_steps"] / max(ev["counters"]["path_steps"], 1),
+                               "note": "above 1: the variance-targeted counts lose to the same paths spread evenly "
+                                       "(the knots with the largest score variance are also those with the longest "
+                                       "paths, and the score's standard error is not what limits the interface error)"},
+        "equation_variance": {"seconds": wall_eq, "rounds": req["rounds"], "paths": req["paths"],
+                              "path_steps": req["path_steps"], "max_se": req["max_se"],
+                              "knots_over": req["knots_over"], "knots_capped": req["knots_capped"],
+                              "paths_per_knot": {"min": int(Neq.min()), "median": int(np.median(Neq)),
+                                                 "max": int(Neq.max())},
+                              "rms_interface_error": float(np.sqrt(np.mean((eq["u"] - uex) ** 2)))},
         "autotune": {"best": {k: getattr(best, k) for k in ("lut_lds", "wg_size", "grid_wgs", "paths_per_lane")},
                      "candidates": tune["candidates"], "best_rate": tune["best_rate"], "base_rate": tune["base_rate"],
                      "ms": tune["total_ms"]},

@@ -115,7 +115,7 @@ def build(name, engine=None, shape=None):
 # radius 1 cut into 6 rings x 24 sectors with 8..64 knots per interface (seed 11); per-knot path
 # counts from 1e3 to 1e6 targeting a standard error of 1e-3 per knot equation; FP64 paths.
 DISC = {"radius": 1.0, "n_rings": 6, "n_sectors": 24, "seed": 11, "kmin": 8, "kmax": 64, "shape": 1.0,
-        "h": 1e-4, "path_seed": 7, "se_target": 2e-3, "n_min": 1000, "n_max": 1000000}
+        "h": 1e-4, "path_seed": 7, "se_target": 1e-3, "n_min": 1000, "n_max": 1000000}
 
 
 def build_disc(engine):

@@ -403,7 +403,8 @@ def batching_bench(eng):
                               "paths_per_knot": {"min": int(Neq.min()), "median": int(np.median(Neq)),
                                                  "max": int(Neq.max())},
                               "rms_interface_error": float(np.sqrt(np.mean((eq["u"] - uex) ** 2)))},
-        "autotune": {"best": {k: getattr(best, k) for k in ("lut_lds", "wg_size", "grid_wgs", "paths_per_lane")},
+        "autotune": {"best": {k: getattr(best, k) for k in ("lut_lds", "wg_size", "grid_wgs", "paths_per_lane",
+                                                                "lut_pad")},
                      "candidates": tune["candidates"], "best_rate": tune["best_rate"], "base_rate": tune["base_rate"],
                      "ms": tune["total_ms"]},
     }

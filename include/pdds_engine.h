@@ -134,6 +134,8 @@ typedef struct {
                                distance to the nearest side (PDDAlgorithms/PDDSparseGM.cpp:4025-4029,
                                :4386-4389).  0: every path steps with h (default).  Not with
                                reflecting sides. */
+    int32_t lut_pad;        /* LDS tile shape: spare texels per tile row (0..15, default 0), which
+                               move the LDS banks each row starts on; results do not depend on it */
 } pdds_run;
 void pdds_run_defaults(pdds_run *r);
 

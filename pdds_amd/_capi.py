@@ -34,7 +34,8 @@ class Run(C.Structure):
                 ("precision", C.c_int32), ("quirk_last_side", C.c_int32), ("max_steps", C.c_int64),
                 ("sub_batch", C.c_int32), ("item_paths", C.c_int32), ("wg_size", C.c_int32),
                 ("grid_wgs", C.c_int32), ("lut_lds", C.c_int32), ("plain", C.c_int32),
-                ("paths_per_lane", C.c_int32), ("step", C.c_int32), ("adaptive_h", C.c_int32)]
+                ("paths_per_lane", C.c_int32), ("step", C.c_int32), ("adaptive_h", C.c_int32),
+                ("lut_pad", C.c_int32)]
 
 
 class Adaptive(C.Structure):

@@ -186,6 +186,7 @@ pdds_status check_run(pdds_engine *e, const pdds_run *r) {
                   "step family and problem values");
     if (r->step < PDDS_STEP_CV || r->step > PDDS_STEP_VRCV) return fail(e, PDDS_ERR_INVALID, "bad step family");
     if (r->adaptive_h != 0 && r->adaptive_h != 1) return fail(e, PDDS_ERR_INVALID, "adaptive_h must be 0 or 1");
+    if (r->lut_pad < 0 || r->lut_pad > 15) return fail(e, PDDS_ERR_INVALID, "lut_pad must be in [0, 15]");
     return PDDS_OK;
 }
 
@@ -284,7 +285,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         for (size_t i = 0; i < knots.size(); ++i) {
             int c0, r0, tw, th;
             lut_tile_of(pp.lut, host_sd(knots[i]).bbox, c0, r0, tw, th);
-            const int fits = (size_t)tw * th * sizeof(float4) <= tile_cap ? 1 : 0;
+            const int fits = (size_t)(tw + r->lut_pad) * th * sizeof(float4) <= tile_cap ? 1 : 0;
             tkey[i] = {c0, r0, tw, th, fits};
             const std::array<int, 4> k4 = fits ? std::array<int, 4>{c0, r0, tw, th} : std::array<int, 4>{-1, -1, -1, -1};
             auto it = first.find(k4);
@@ -392,7 +393,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
                     pd.tiled = tk[4];
                     if (tk[4]) {
                         pd.c0 = tk[0]; pd.r0 = tk[1]; pd.tw = tk[2]; pd.th = tk[3];
-                        lds_bytes = std::max(lds_bytes, (size_t)tk[2] * tk[3] * sizeof(float4));
+                        pd.pitch = tk[2] + r->lut_pad;
+                        lds_bytes = std::max(lds_bytes, (size_t)pd.pitch * tk[3] * sizeof(float4));
                     }
                     patches.push_back(pd);
                 }
@@ -1183,13 +1185,14 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
     }
     // candidate launch shapes (none of them changes a result bit)
     std::vector<pdds_run> cands;
-    auto add = [&](int lds, int wg, int grid, int ppl) {
+    auto add = [&](int lds, int wg, int grid, int ppl, int pad) {
         pdds_run r = *base;
         r.plain = 0;
         r.lut_lds = lds;
         r.wg_size = wg;
         r.grid_wgs = grid;
         r.paths_per_lane = ppl;
+        r.lut_pad = pad;
         cands.push_back(r);
     };
     {
@@ -1197,13 +1200,15 @@ pdds_status pdds_autotune(pdds_engine *e, const pdds_mesh *m, const pdds_problem
         r.plain = 0;
         cands.push_back(r);
     }
+    // LDS-tiled shapes: work-groups per CU, paths per lane and the tile's row pitch
     const bool can_tile = p->pde == PDDS_PDE_VARCOEF_LUT && pp.lut.nf == 5;
     if (can_tile)
-        for (int g : {1, 2})
-            for (int ppl : {1, 2, 4, 8}) add(1, 0, g * e->n_cu, ppl);
+        for (int pad : {0, 1, 2, 3})
+            for (int g : {1, 2})
+                for (int ppl : {1, 2, 4, 8}) add(1, 0, g * e->n_cu, ppl, pad);
     for (int wg : {64, 128, 256})
         for (int g : {1, 2, 4, 8})
-            for (int ppl : {1, 4, 16}) add(0, wg, g * e->n_cu * 256 / wg, ppl);
+            for (int ppl : {1, 4, 16}) add(0, wg, g * e->n_cu * 256 / wg, ppl, 0);
     double best_rate = -1.0, base_rate = 0.0;
     size_t best_i = 0;
     for (size_t i = 0; i < cands.size(); ++i) {

@@ -664,12 +664,12 @@ path_kernel_tiled(PathKernelArgs a) {
         p = __builtin_amdgcn_readfirstlane(s_patch);   // uniform: the patch, its tile and queue stay scalar
         if (p < 0) break;
         const PatchDesc pd = a.patches[p];
-        const LutTile T{(lds_float4 *)tile, pd.c0, pd.r0, pd.tw, pd.tiled ? pd.th : 0};
+        const LutTile T{(lds_float4 *)tile, pd.c0, pd.r0, pd.tw, pd.tiled ? pd.th : 0, pd.pitch};
         if (pd.tiled) {
             const int nt = pd.tw * pd.th;
             for (int i = threadIdx.x; i < nt; i += blockDim.x) {
                 const int r = i / pd.tw, c = i - r * pd.tw;
-                tile[i] = a.pp.lut.a[(int64_t)(pd.r0 + r) * a.pp.lut.px + (pd.c0 + c)];
+                tile[r * pd.pitch + c] = a.pp.lut.a[(int64_t)(pd.r0 + r) * a.pp.lut.px + (pd.c0 + c)];
             }
         }
         __syncthreads();

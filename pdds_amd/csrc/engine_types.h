@@ -108,7 +108,7 @@ struct Records {
 struct PatchDesc {
     int64_t pbeg, pend;
     int32_t c0, r0, tw, th;
-    int32_t tiled, pad;
+    int32_t tiled, pitch;   // pitch: texels per LDS tile row (tw plus the run's lut_pad)
 };
 
 struct PathKernelArgs {

@@ -516,9 +516,10 @@ PDDS_HD const T &lut_at(const void *base, uint32_t byte_off) {
 }
 
 // A rectangle of plane a held in LDS: padded columns [c0, c0 + tw), rows [r0, r0 + th),
-// row-major with row length tw.  th == 0 means no tile.  The pointer is typed in the LDS
-// address space, so tile reads are ds_read_b128 and never merge with the global-memory
-// branch into generic (flat) loads.
+// row-major with `pitch` texels per row (pitch >= tw; the spare texels change which LDS banks
+// the rows start on).  th == 0 means no tile.  The pointer is typed in the LDS address space,
+// so tile reads are ds_read_b128 and never merge with the global-memory branch into generic
+// (flat) loads.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(3))) const float4 lds_float4;
 #else
@@ -527,6 +528,7 @@ typedef const float4 lds_float4;   // the host pass never dereferences a tile
 struct LutTile {
     lds_float4 *lds;
     int c0, r0, tw, th;
+    int pitch;
 };
 
 // The 16-tap accumulation of lut_eval_t.  SRC 0: plane a from global memory; 1: from the LDS
@@ -553,7 +555,7 @@ PDDS_HD void lut_accum(const LutView &L, const LutTile &T, lds_float4 *tp, bool 
         float4 A[4];
         if (SRC == 1 || (SRC == 2 && in)) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) A[a] = tp[b * T.tw + a];
+            for (int a = 0; a < 4; ++a) A[a] = tp[b * T.pitch + a];
         } else {
 #pragma unroll
             for (int a = 0; a < 4; ++a) A[a] = lut_at<float4>(L.a, (r0 + (uint32_t)a) * 16u);
@@ -614,7 +616,7 @@ PDDS_HD void lut_eval_t(const LutView &L, const LutTile &T, R x, R y, R out[NF])
     if constexpr (TILED) {
         const int lc = pc - T.c0, lr = pr - T.r0;
         const bool in = lc >= 0 && lr >= 0 && lc + 3 < T.tw && lr + 3 < T.th;
-        lds_float4 *tp = T.lds + (in ? lr * T.tw + lc : 0);
+        lds_float4 *tp = T.lds + (in ? lr * T.pitch + lc : 0);
 #if defined(__HIP_DEVICE_COMPILE__)
         if (__all(in)) lut_accum<NF, 1>(L, T, tp, in, t0, wx, wy, acc);
         else lut_accum<NF, 2>(L, T, tp, in, t0, wx, wy, acc);

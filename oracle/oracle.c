@@ -588,6 +588,38 @@ static inline double orc_rbf_ic2(double px, double py, double x, double y, doubl
     return orc_exp_tab(-(dx * dx + dy * dy) * inv_c2);
 }
 
+/* Gaussian RBF values phi_j = exp(-|p_j - X|^2 / c^2) of the entries j of one straight,
+   uniformly spaced stencil side (a rectangle side), chunk by chunk of ORC_RBF_CHUNK entries: the
+   chunk's first entry j0 directly, the others by the two-multiply recurrence
+   phi_{k+1} = phi_k r_k, r_{k+1} = r_k q, where with the side's step s (its first two entries)
+   and d = p_j0 - X, q = exp(-2 |s|^2 / c^2) and r_0 = exp(-(2 d.s + |s|^2) / c^2) (above 1 while
+   the entries approach X; its exponent is capped at 700, so an underflowed start stays 0).
+   Replaces one exponential per entry (G_update's phi, Meshes/stencil.cpp:246-256) by two per chunk
+   and one per side; the values equal the direct ones to about 1e-13 relative (values below
+   about 1e-180 may come out as 0).  The engine's reduction evaluates exactly these operations. */
+#define ORC_RBF_CHUNK 16
+static void orc_rbf_side(const double *px, const double *py, int m, double x, double y, double ic2, double *phi) {
+    double sx = 0.0, sy = 0.0, ss = 1.0;
+    if (m > 1) {
+        sx = px[1] - px[0];
+        sy = py[1] - py[0];
+        ss = sx * sx + sy * sy;
+    }
+    const double q = orc_exp_tab(-(2.0 * ss) * ic2);
+    for (int j0 = 0; j0 < m; j0 += ORC_RBF_CHUNK) {
+        const int j1 = j0 + ORC_RBF_CHUNK < m ? j0 + ORC_RBF_CHUNK : m;
+        const double dx = px[j0] - x, dy = py[j0] - y;
+        double v = orc_exp_tab(-(dx * dx + dy * dy) * ic2);
+        double r = orc_exp(fmin(-(2.0 * (dx * sx + dy * sy) + ss) * ic2, 700.0));
+        phi[j0] = v;
+        for (int j = j0 + 1; j < j1; ++j) { v = v * r; phi[j] = v; r = r * q; }
+    }
+}
+
+void orc_rbf_side_eval(const double *px, const double *py, int m, double x, double y, double ic2, double *phi) {
+    orc_rbf_side(px, py, m, x, y, ic2, phi);
+}
+
 /* CSR row of a stencil: sorted unique columns; for each side entry its column slot. */
 static int orc_row_pattern(const int msz[4], const int *idx, int *cols, int *slot) {
     int tot = msz[0] + msz[1] + msz[2] + msz[3];
@@ -739,6 +771,7 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
     double ic2[4];
     for (int s = 0; s < 4; ++s) ic2[s] = 1.0 / (cshape[s] * cshape[s]);
     double *Phi = (double *)calloc((size_t)tot + 1, sizeof(double));
+    double *phis = (double *)calloc((size_t)tot + 1, sizeof(double));
     double *Gb = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *isum = (double *)calloc((size_t)nnz + 1, sizeof(double));
     double *isq = (double *)calloc((size_t)nnz + 1, sizeof(double));
@@ -767,7 +800,15 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
                     cnt[4]++;
                 } else {
                     if (side < 0) cnt[5]++;
-                    else {
+                    else if (geom == 0) {
+                        /* rectangle sides: straight and uniformly spaced, one Gaussian width (the
+                           side's, or its first entry's when widths are given per entry) */
+                        cnt[side]++;
+                        const double ics = cent ? 1.0 / (cent[off[side]] * cent[off[side]]) : ic2[side];
+                        orc_rbf_side(px + off[side], py + off[side], msz[side], xt[n], yt[n], ics, phis);
+                        for (int j = 0; j < msz[side]; ++j)
+                            Phi[off[side] + j] = fma(Y[n], phis[j], Phi[off[side] + j]);
+                    } else {
                         cnt[side]++;
                         for (int j = 0; j < msz[side]; ++j)
                             Phi[off[side] + j] = fma(Y[n],
@@ -820,7 +861,7 @@ int orc_reduce_batched_geom(const orc_problem *pb, int64_t npaths, int64_t S, in
         *eq_mean = rs / N;
         *eq_var = nb > 1 ? (rq - N * *eq_mean * *eq_mean) / (double)(nb - 1) : 0.0;
     }
-    free(slot); free(use); free(Phi); free(Gb); free(isum); free(isq); free(sum); free(sq);
+    free(slot); free(use); free(Phi); free(phis); free(Gb); free(isum); free(isq); free(sum); free(sq);
     return nnz;
 }
 

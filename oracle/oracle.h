@@ -83,6 +83,9 @@ typedef struct {
 /* ---- elementary pieces (for tests) ---- */
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_math_eval(int fn, const double *x, double *y, int64_t n);   /* 0 exp,1 log,2 sin,3 cos,4 tanh,5 sinpi,6 cospi,7 expm1,8 table exp (x <= 0) */
+/* Gaussian RBF values of the m entries of a straight, uniformly spaced stencil side at X, as the
+   batched reduction forms them on rectangle sides (chunked two-multiply recurrence) */
+void orc_rbf_side_eval(const double *px, const double *py, int m, double x, double y, double ic2, double *phi);
 void orc_normals(const orc_run *run, uint32_t knot, uint32_t path, int64_t step0, int64_t nsteps, double *n1, double *n2);
 double orc_rect_dist(const double p[4], double x, double y, double *ex, double *ey, double *nx, double *ny);
 double orc_sector_dist(const double g[6], double x, double y, double *ex, double *ey, double *nx, double *ny,

@@ -86,6 +86,7 @@ def lib():
         L.orc_math_eval.argtypes = [C.c_int, _d, _d, C.c_int64]
         L.orc_normals.argtypes = [C.POINTER(Run), C.c_uint32, C.c_uint32, C.c_int64, C.c_int64, _d, _d]
         L.orc_lut_prefilter.argtypes = [C.c_int, C.c_int, C.c_int, _d, _f]
+        L.orc_rbf_side_eval.argtypes = [_d, _d, C.c_int, C.c_double, C.c_double, C.c_double, _d]
         _LIB = L
     return _LIB
 
@@ -110,6 +111,15 @@ def math_eval(fn, x):
     lib().orc_math_eval({"exp": 0, "log": 1, "sin": 2, "cos": 3, "tanh": 4, "sinpi": 5, "cospi": 6,
                          "expm1": 7, "exp_tab": 8}[fn], ptr(x), ptr(y), x.size)
     return y
+
+
+def rbf_side(px, py, x, y, ic2):
+    """The reduction's Gaussian RBF values of one rectangle side's entries at X (recurrence form)."""
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    out = np.zeros(len(px))
+    lib().orc_rbf_side_eval(ptr(px), ptr(py), len(px), x, y, ic2, ptr(out))
+    return out
 
 
 def rect_dist(p, x, y):

@@ -452,6 +452,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         ra.knots = dk; ra.tables = dtab; ra.items = dit; ra.rec = rec;
         ra.stats = dstats; ra.partials = dpart; ra.ne_cap = ne_cap; ra.nnz_cap = nnz_cap; ra.S = S;
         ra.wref = dW;
+        ra.side_rbf = geom == GeomRect ? 1 : 0;   // rectangle sides: the chunked recurrence
         const int64_t nit = (int64_t)items.size();
         for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
             ra.item_base = base;

@@ -219,6 +219,46 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         }
         __syncthreads();
         // moments Phi[e][b] = sum over the exits of e's side in sub-batch b of Y phi_e(X)
+        if (a.side_rbf) {
+            // rectangle sides: a thread per (sub-batch, side, chunk of kRbfChunk entries) walks the
+            // side's exits of the sub-batch in path order, forms the chunk's RBF values of each
+            // (rbf_side_chunk) and adds Y phi into one register sum per entry -- per entry the
+            // order of the direct form
+            int cb[5];
+            cb[0] = 0;
+            for (int s = 0; s < 4; ++s) cb[s + 1] = cb[s] + (sd.msz[s] + kRbfChunk - 1) / kRbfChunk;
+            for (int t = tid; t < nb * cb[4]; t += WG) {
+                const int b = t / cb[4], r = t - b * cb[4];
+                int s = 0;
+                while (r >= cb[s + 1]) ++s;
+                const int m = sd.msz[s];
+                const int j0 = (r - cb[s]) * kRbfChunk;
+                const int len = (m - j0) < kRbfChunk ? (m - j0) : kRbfChunk;
+                const int e0 = sd.ent_off + eoff[s];
+                const double *px = tb.entx + e0, *py = tb.enty + e0;
+                const double ic2 = tb.entic2[e0];
+                double sx = 0.0, sy = 0.0, ss = 1.0;
+                if (m > 1) {
+                    sx = px[1] - px[0];
+                    sy = py[1] - py[0];
+                    ss = sx * sx + sy * sy;
+                }
+                const double q = dexp_tab_nonpos(-(2.0 * ss) * ic2, s_exp2);
+                const uint16_t *L = lst + (b * 4 + s) * S;
+                const int nl = s_lcnt[b * 4 + s];
+                const int pb = b * S;
+                double acc[kRbfChunk];
+#pragma unroll
+                for (int k = 0; k < kRbfChunk; ++k) acc[k] = 0.0;
+                for (int l = 0; l < nl; ++l) {
+                    const int p = pb + L[l];
+                    rbf_side_chunk(px, py, j0, sx, sy, ss, q, gx[p], gy[p], ic2, s_exp2, gY[p], acc);
+                }
+#pragma unroll
+                for (int k = 0; k < kRbfChunk; ++k)
+                    if (k < len) Phi[(eoff[s] + j0 + k) * kNBG + b] = acc[k];
+            }
+        } else
         for (int e = tid; e < ne; e += WG) {
             const int se = tb.ents[sd.ent_off + e];
             const double pxe = tb.entx[sd.ent_off + e], pye = tb.enty[sd.ent_off + e];
@@ -354,6 +394,11 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
 }
 
 hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
+    if (shmem > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(a.wref ? (const void *)reduce_kernel<true> : (const void *)reduce_kernel<false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+    }
     if (a.wref)
         hipLaunchKernelGGL(reduce_kernel<true>, dim3(grid), dim3(wg), shmem, s, a);
     else

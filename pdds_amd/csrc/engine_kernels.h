@@ -85,6 +85,28 @@ PDDS_HD double rbf_tab(double px, double py, double x, double y, double inv_c2, 
     return dexp_tab_nonpos(-(dx * dx + dy * dy) * inv_c2, T);
 }
 
+// The same Gaussian at exit (x, y) for the kRbfChunk entries from j0 of a straight, uniformly
+// spaced stencil side (px, py; a rectangle side) with step (sx, sy), ss = |s|^2 and
+// q = exp(-2 ss / c^2): entry j0 directly, the others by the two-multiply recurrence
+// phi_{k+1} = phi_k r_k, r_{k+1} = r_k q with d = p_j0 - X and r_0 = exp(-(2 d.s + ss) / c^2)
+// (above 1 while the entries approach X; its exponent capped at 700, so an underflowed start
+// stays 0).  Two exponentials per chunk instead of one per entry; the values equal the direct ones
+// to about 1e-13 relative (those below about 1e-180 may come out as 0).  Each value phi_k is added
+// as acc[k] = fma(Y, phi_k, acc[k]); the sums for k past the side's end are formed and not used.
+PDDS_HD void rbf_side_chunk(const double *px, const double *py, int j0, double sx, double sy, double ss, double q,
+                            double x, double y, double inv_c2, const double *T, double Y, double acc[kRbfChunk]) {
+    const double dx = px[j0] - x, dy = py[j0] - y;
+    double p = dexp_tab_nonpos(-(dx * dx + dy * dy) * inv_c2, T);
+    double r = dexp(__builtin_fmin(-(2.0 * (dx * sx + dy * sy) + ss) * inv_c2, 700.0));
+    acc[0] = __builtin_fma(Y, p, acc[0]);
+#pragma unroll
+    for (int k = 1; k < kRbfChunk; ++k) {
+        p = p * r;
+        acc[k] = __builtin_fma(Y, p, acc[k]);
+        r = r * q;
+    }
+}
+
 // ----------------------------------------------------------------------------
 // Per-lane path state
 // ----------------------------------------------------------------------------

@@ -7,6 +7,7 @@ namespace pdds {
 
 constexpr int kNBG = 8;                    // sub-batches reduced per pass over Psi^-1
 constexpr int kReduceWG = 256;            // reduction work-group
+constexpr int kRbfChunk = 16;             // side entries per direct exponential (rectangle sides)
 
 // LDS of the reduction kernel: Phi [ne_cap][kNBG], accS/accQ [nnz_cap] (FP64), the group's
 // records x, y, Y and score [kNBG*S] (FP64), their sides [kNBG*S] (int8) and per-(sub-batch,
@@ -147,6 +148,9 @@ struct ReduceArgs {
     const double *wref;
     int32_t ne_cap, nnz_cap;
     int32_t S;
+    // 1: every stencil side is straight and uniformly spaced with one Gaussian width (rectangle
+    // meshes), and the RBF values of a side come from rbf_side_chunk
+    int32_t side_rbf;
 };
 
 struct FinalizeArgs {

@@ -28,6 +28,12 @@ import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr as each part of the run starts or ends (stdout carries only the JSON)."""
+    print("[bench %7.1f s] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
 
 
 def kernel_source_hash():
@@ -295,10 +301,12 @@ def serving_bench(eng, duration_s=1.5, h=5e-5, seed=23, plain=True):
         else:
             tickets.append(srv.submit_solve(r["n_sub"], r["n_node"], fac[(r["n_sub"], r["n_node"])], r["n"],
                                             k2=r["k2"], stream=i + 1, h=r["h"]))
+    progress("serving: %d requests submitted" % len(reqs))
     res = [srv.wait(t, cap=1 << 20) for t in tickets]
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
+    progress("serving: all requests done in %.2f s" % wall)
     lat = {"point": [x["latency_ms"] for x, r in zip(res, reqs) if r["kind"] == "point"],
            "solve": [x["latency_ms"] for x, r in zip(res, reqs) if r["kind"] == "solve"]}
     u_exact = bvp("helmholtz").u
@@ -342,6 +350,8 @@ def serving_bench(eng, duration_s=1.5, h=5e-5, seed=23, plain=True):
                 eng.solve(m.n_knots, m.row_ptr, m.cols, o["G"], o["B"], tol=1e-10)
             steps += o["counters"]["path_steps"]
             lat_p[r["kind"]].append(1e3 * (time.perf_counter() - (t0 + r["t"])))
+            if (i + 1) % 50 == 0:
+                progress("serving, plain FP64 one at a time: %d of %d requests" % (i + 1, len(reqs)))
         wall_p = time.perf_counter() - t0
         out["plain_gpu"] = {"latency_ms": {"p50": _pct(lat_p["point"] + lat_p["solve"], 50),
                                            "p99": _pct(lat_p["point"] + lat_p["solve"], 99),
@@ -361,7 +371,9 @@ def batching_bench(eng):
     mesh, prob, h, seed, cfg = W.build_disc(eng)
     t_mesh = time.perf_counter() - t0
     base = default_run(h=h, seed=seed)
+    progress("batching: mesh built (%d knots), autotuning" % mesh.n_knots)
     best, tune = eng.autotune(mesh, prob, base)
+    progress("batching: variance-targeted counts")
     t1 = time.perf_counter()
     out = eng.solve_adaptive(mesh, prob, best, **cfg)
     wall = time.perf_counter() - t1
@@ -372,11 +384,13 @@ def batching_bench(eng):
     err = out["u"] - uex
     # the same number of paths spread evenly over the knots, one assembly and solve
     n_even = max(1, int(round(rep["paths"] / mc.sum())))
+    progress("batching: even counts (%d paths per knot)" % n_even)
     t2 = time.perf_counter()
     ev = eng.assemble(mesh, prob, best, n_even)
     u_even = eng.solve(mesh.n_knots, mesh.row_ptr, mesh.cols, ev["G"], ev["B"], tol=1e-10)[0]
     t_even = time.perf_counter() - t2
     # the equation-variance rule at the same target (the knot equation B_k - sum_j G_kj u_j)
+    progress("batching: equation-variance counts")
     t3 = time.perf_counter()
     eq = eng.solve_adaptive(mesh, prob, best, variance=1, **cfg)
     wall_eq = time.perf_counter() - t3
@@ -472,6 +486,7 @@ def main():
         n_paths = args.paths
     run = default_run(h=h, seed=seed, precision=args.precision)
 
+    progress("%s: %d warmup + %d timed assemblies, %s paths" % (args.config, args.warmup, args.steps, args.precision))
     for _ in range(args.warmup):
         eng.assemble(mesh, prob, run, n_paths)
     steps = 0
@@ -486,6 +501,7 @@ def main():
         kern_ms.append(out["counters"]["kernel_ms"])
     wall = time.perf_counter() - t0
     value = steps / wall
+    progress("timed assemblies done: %.1f ms per assembly" % (1e3 * wall / args.steps))
     rms, iters = rms_error(eng, mesh, prob, out)
     c = out["counters"]
     avg_kernel_s = float(np.mean(kern_ms)) / 1e3
@@ -550,6 +566,7 @@ def main():
     if not args.no_extras:
         extras = {}
         for name, kw in (("plain_fp64", dict(plain=1)), ("engine_fp32", dict(precision="fp32"))):
+            progress("form %s" % name)
             r2 = default_run(h=h, seed=seed, **kw)
             ts = time.perf_counter()
             o2 = eng.assemble(mesh, prob, r2, n_paths)
@@ -563,6 +580,7 @@ def main():
         # and on the plain FP64 kernel; on this workload the RMS error is set by the time step and
         # the RBF interpolation (it is the same from 1e3 to 1e5 paths per knot), not by the counts
         for name, kw in (("adaptive_fp32", dict(precision="fp32")), ("adaptive_plain_fp64", dict(plain=1))):
+            progress("form %s" % name)
             ts = time.perf_counter()
             ad = eng.solve_adaptive(mesh, prob, default_run(h=h, seed=seed, **kw), se_target=4e-3,
                                     n_min=1000, n_max=1000000, max_rounds=4)
@@ -576,10 +594,13 @@ def main():
                                                             / extras["adaptive_fp32"]["seconds"])
         result["forms"] = extras
     if not args.no_serving and not args.no_extras:
+        progress("serving scenario")
         result["serving"] = serving_bench(eng, args.serving_seconds)
     if not args.no_extras and not args.no_batching:
+        progress("batching scenario")
         result["batching"] = batching_bench(eng)
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(mesh, {3: "varcoef_lut", 0: "monegros", 1: "sinsin"}[prob.pde],
                                               n_paths, h, seed)
     eng.close()

@@ -285,10 +285,11 @@ def serving_bench(eng, duration_s=1.5, h=5e-5, seed=23, plain=True):
     for r in reqs:
         if r["kind"] == "solve":
             fac[(r["n_sub"], r["n_node"])] = relative_fac(dom[:2], dom[2:], r["n_sub"], r["n_node"], 0.8)
-    # one preparation thread per host core available (at most 16): a solve's mesh and Psi^-1
-    # tables are built on the host, and with 8 threads that preparation kept the device idle for
-    # a fifth of the stream (118 against 143 G path-steps/s with 16)
-    prep = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    # two preparation threads: a solve's mesh and Psi^-1 tables are built on the host, each build
+    # spread over the host cores by itself, so two workers keep the cores busy and preparation
+    # close to arrival order (16 workers ran 16 builds at once, finished them all late and left
+    # the device idle for a quarter of the stream: 117 against 167 G path-steps/s with 2)
+    prep = 2
     srv = Server(eng, bvp("helmholtz"), default_run(h=h, seed=seed, precision="fp32"), dom, prep_threads=prep)
     t0 = time.perf_counter()
     tickets = []

@@ -367,7 +367,9 @@ typedef struct {
                                its reduction partials and descriptors, and its G values -- an
                                estimate of what it adds to the engine's buffers */
     int32_t max_inflight;   /* requests admitted at once (0: 256) */
-    int32_t prep_threads;   /* host threads preparing requests (0: 4) */
+    int32_t prep_threads;   /* host threads preparing requests (0: 2); a solve's mesh build already
+                               spreads over up to 16 host threads, so more workers mostly share
+                               the same cores and delay every request alike */
     int64_t round_paths;    /* paths per round (0: 2^25) */
 } pdds_server_config;
 

@@ -493,6 +493,8 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
                  std::vector<double> &enty, std::vector<int8_t> &ents, std::vector<double> &entic2,
                  std::vector<int32_t> &e1,
                  std::vector<int32_t> &e2, std::vector<double> &ipsi) {
+    // one Psi^-1 block per distinct side: stencils that share a side point at the same block
+    std::vector<int64_t> side_off;
     for (const Stencil &s : m.stencils) {
         StencilDesc d;
         std::memset(&d, 0, sizeof(d));
@@ -508,8 +510,17 @@ void mesh_tables(const Mesh &m, int quirk, std::vector<StencilDesc> &sten, std::
         d.csr_off = (int32_t)e1.size();
         d.gmask = s.gmask;
         for (int i = 0; i < 4; ++i) {
-            d.ipsi_off[i] = (int64_t)ipsi.size();
-            ipsi.insert(ipsi.end(), s.ipsi[i].begin(), s.ipsi[i].end());
+            const int sid = s.side_id[i];
+            if (sid >= 0 && sid < (int)side_off.size() && side_off[sid] >= 0) {
+                d.ipsi_off[i] = side_off[sid];
+            } else {
+                d.ipsi_off[i] = (int64_t)ipsi.size();
+                ipsi.insert(ipsi.end(), s.ipsi[i].begin(), s.ipsi[i].end());
+                if (sid >= 0) {
+                    if (sid >= (int)side_off.size()) side_off.resize(sid + 1, -1);
+                    side_off[sid] = d.ipsi_off[i];
+                }
+            }
             for (size_t j = 0; j < s.idx[i].size(); ++j) {
                 entx.push_back(s.px[i][j]);
                 enty.push_back(s.py[i][j]);

@@ -184,11 +184,37 @@ void Mesh::finalize(std::vector<Stencil> &per_knot, const std::vector<char> &has
         stencil_of[q] = it->second;
     }
     per_knot.clear();
-    // per stencil side: Psi^-1 (independent dense inversions, spread over host threads)
+    // per distinct stencil side: Psi^-1 (independent dense inversions, spread over host threads).
+    // The same interface segments bound several stopping rectangles; a side whose entry
+    // positions, Gaussian widths and conditioning limit equal an earlier side's bit for bit has
+    // the same Psi, so its inverse (and any shape shrink) is computed once and copied.
     std::vector<std::pair<int, int>> jobs;
-    for (int t = 0; t < (int)stencils.size(); ++t)
-        for (int d = 0; d < 4; ++d)
-            if (!stencils[t].idx[d].empty()) jobs.emplace_back(t, d);
+    std::vector<std::pair<std::pair<int, int>, int>> dups;
+    {
+        std::map<std::vector<uint64_t>, int> side_job;
+        for (int t = 0; t < (int)stencils.size(); ++t)
+            for (int d = 0; d < 4; ++d) {
+                const Stencil &s = stencils[t];
+                if (s.idx[d].empty()) continue;
+                std::vector<uint64_t> key;
+                key.reserve(3 + 3 * s.px[d].size());
+                key.push_back(f64_bits(s.cshape[d]));
+                key.push_back(f64_bits(s.cond_max));
+                key.push_back(s.px[d].size());
+                for (double v : s.px[d]) key.push_back(f64_bits(v));
+                for (double v : s.py[d]) key.push_back(f64_bits(v));
+                for (double v : s.cent[d]) key.push_back(f64_bits(v));
+                auto it = side_job.find(key);
+                if (it == side_job.end()) {
+                    stencils[t].side_id[d] = (int)jobs.size();
+                    side_job.emplace(std::move(key), (int)jobs.size());
+                    jobs.emplace_back(t, d);
+                } else {
+                    stencils[t].side_id[d] = it->second;
+                    dups.push_back({{t, d}, it->second});
+                }
+            }
+    }
     std::vector<char> ok(jobs.size(), 1);
     std::atomic<size_t> next{0};
     auto work = [&]() {
@@ -210,6 +236,17 @@ void Mesh::finalize(std::vector<Stencil> &per_knot, const std::vector<char> &has
     for (size_t i = 1; i < nth; ++i) pool.emplace_back(work);
     work();
     for (std::thread &th : pool) th.join();
+    for (const auto &dj : dups) {
+        const Stencil &src = stencils[jobs[dj.second].first];
+        const int ds = jobs[dj.second].second;
+        Stencil &s = stencils[dj.first.first];
+        const int d = dj.first.second;
+        s.cshape[d] = src.cshape[ds];
+        s.cent[d] = src.cent[ds];
+        s.ipsi[d] = src.ipsi[ds];
+        s.cond[d] = src.cond[ds];
+        if (!ok[dj.second]) { s.singular = true; any_singular = true; }
+    }
     for (size_t j = 0; j < jobs.size(); ++j)
         if (!ok[j]) { stencils[jobs[j].first].singular = true; any_singular = true; }
     // per stencil: CSR row and column -> side entry maps

@@ -49,6 +49,9 @@ struct Stencil {
     std::vector<double> cent[4];           // per-entry Gaussian widths (empty: cshape of the side)
     double cond[4] = {0, 0, 0, 0};
     std::vector<double> ipsi[4];          // row-major m x m
+    // distinct-side index within the mesh: sides with the same index have the same entries,
+    // widths and Psi^-1 bit for bit (-1: empty side)
+    int side_id[4] = {-1, -1, -1, -1};
     std::vector<int> cols;                 // sorted unique CSR columns
     std::vector<int> e1, e2;               // per column: contributing side entries (concatenated N,S,E,W), -1 none
     std::vector<int> e1_sum, e2_sum;       // same, summing shared corners

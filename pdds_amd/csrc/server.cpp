@@ -4,13 +4,16 @@
 // pdds_solve / pdds_point_eval.
 //
 // Threads: prep_threads host workers prepare requests (a solve builds its mesh, Psi^-1 and
-// host stencil tables, in parallel over its stencils); one device thread runs rounds.  A round
+// host stencil tables, each build spread over up to 16 host threads of its own, so a few
+// workers already occupy the host and keep preparation close to arrival order); one device
+// thread runs rounds; a solver thread runs the interface solves.  A round
 // admits prepared requests first-fit by device footprint, builds one knot list over every
 // admitted request (point paths first, then the next knots of each solve, up to round_paths
 // paths), runs it as one assembly over the requests' stencil tables, scatters the per-knot
-// results back to their requests, solves (I + G) u = B for each solve whose knots are all done,
-// and completes requests.  Each request's paths are keyed by its own Philox stream and every
-// sum has a fixed order, so its results do not depend on the other requests of a round.
+// results back to their requests, completes the points and hands each solve whose knots are
+// all done to the solver thread, which solves (I + G) u = B on a second engine.  Each
+// request's paths are keyed by its own Philox stream and every sum has a fixed order, so its
+// results do not depend on the other requests of a round.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -378,7 +381,7 @@ pdds_status pdds_server_create(pdds_engine *e, const pdds_server_config *cfg, pd
     if (!(c.domain[2] > c.domain[0]) || !(c.domain[3] > c.domain[1])) return fail(e, PDDS_ERR_INVALID, "bad domain");
     if (c.hbm_budget <= 0) c.hbm_budget = (int64_t)16 << 30;
     if (c.max_inflight <= 0) c.max_inflight = 256;
-    if (c.prep_threads <= 0) c.prep_threads = 4;
+    if (c.prep_threads <= 0) c.prep_threads = 2;
     if (c.round_paths <= 0) c.round_paths = (int64_t)1 << 25;
     std::unique_ptr<pdds_server> s(new pdds_server());
     s->e = e;

@@ -120,6 +120,40 @@ def test_psi_inverse_against_oracle():
             off += n
 
 
+def test_psi_inverse_every_side_of_config2():
+    """The 8x8 x 32-knot mesh computes Psi^-1 once per distinct side (the same interface segments
+    bound several stopping rectangles) and copies it to the stencils sharing that side: every side
+    of every distinct stencil must hold the inverse of its own Psi, with its own condition number."""
+    from pdds_amd.workloads import relative_fac
+    fac = relative_fac((-1.0, -1.0), (1.0, 1.0), 8, 32, 0.8)
+    m = _mesh((-1.0, -1.0), (1.0, 1.0), 8, 32, fac)
+    seen, sides, shared = set(), 0, {}
+    for k in np.where(~m.on_boundary)[0]:
+        st = m.stencil(int(k))
+        key = st["rect"].tobytes() + st["msz"].tobytes()
+        if key in seen:
+            continue
+        seen.add(key)
+        off = 0
+        for s in range(4):
+            n = int(st["msz"][s])
+            if n == 0:
+                continue
+            px, py, c = st["px"][off:off + n], st["py"][off:off + n], st["cshape"][s]
+            off += n
+            stat, ip, cond = O.stencil_ipsi(px, py, c)
+            assert stat == 0
+            assert abs(cond - st["cond"][s]) / cond < 1e-6
+            psi = np.exp(-((px[:, None] - px[None, :]) ** 2 + (py[:, None] - py[None, :]) ** 2) / c ** 2)
+            assert np.abs(psi @ st["ipsi"][s] - np.eye(n)).max() < cond * 1e-15
+            sk = px.tobytes() + py.tobytes()
+            if sk in shared:
+                assert np.array_equal(shared[sk], st["ipsi"][s])
+            shared[sk] = st["ipsi"][s]
+            sides += 1
+    assert len(seen) == 161 and sides == 556 and len(shared) == 210
+
+
 def test_center_knot_of_2x2_has_no_stencil():
     m = _mesh((-1.0, -1.0), (1.0, 1.0), 2, 16, [0.25] * 4)
     assert m.n_knots == 61

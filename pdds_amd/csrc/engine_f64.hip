@@ -171,22 +171,45 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     const int ne = sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3];
     const int nnz = sd.csr_nnz;
     const int nsb = (it.n + S - 1) / S;
-    int eoff[5];
-    eoff[0] = 0;
-    for (int q = 0; q < 4; ++q) eoff[q + 1] = eoff[q] + sd.msz[q];
+    // per-side sizes, first entries and Psi^-1 offsets as scalars picked by selects: a private
+    // array indexed by a per-lane side would live in scratch memory
+    const int ms0 = sd.msz[0], ms1 = sd.msz[1], ms2 = sd.msz[2], ms3 = sd.msz[3];
+    const int64_t io0 = sd.ipsi_off[0], io1 = sd.ipsi_off[1], io2 = sd.ipsi_off[2], io3 = sd.ipsi_off[3];
+    auto pick = [](int s, auto v0, auto v1, auto v2, auto v3) { return s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3; };
+    auto msz_of = [&](int s) { return pick(s, ms0, ms1, ms2, ms3); };
+    auto eoff_of = [&](int s) { return pick(s, 0, ms0, ms0 + ms1, ms0 + ms1 + ms2); };
+    auto ipsi_of = [&](int s) { return pick(s, io0, io1, io2, io3); };
     for (int c = tid; c < nnz; c += WG) { accS[c] = 0.0; accQ[c] = 0.0; }
     double Bs = 0.0, BBs = 0.0, Rs = 0.0, RQs = 0.0;
+    // a group's records fit one per thread (GS <= WG): each thread loads its record of the next
+    // group while the current group is reduced, so the loads' latency overlaps the work
+    const bool one = GS <= WG;
+    ExitRec nrc{0.0, 0.0, 0.0, 0.0};
+    int32_t nmt = 0;
+    if (one && tid < GS && tid < it.n) { nrc = rr[tid]; nmt = rmeta[tid]; }
     for (int g = 0; g < nsb; g += kNBG) {
         const int nb = (nsb - g) < kNBG ? (nsb - g) : kNBG;
         const int q0 = g * S;                                   // first path of the group
         const int gn = (it.n - q0) < nb * S ? (it.n - q0) : nb * S;
-        for (int p = tid; p < gn; p += WG) {
-            const ExitRec rc = rr[q0 + p];
-            gx[p] = rc.x;
-            gy[p] = rc.y;
-            gY[p] = rc.Y;
-            gs[p] = rc.score;
-            gside[p] = (int8_t)((rmeta[q0 + p] >> kSideShift) - 1);
+        if (one) {
+            if (tid < gn) {
+                gx[tid] = nrc.x;
+                gy[tid] = nrc.y;
+                gY[tid] = nrc.Y;
+                gs[tid] = nrc.score;
+                gside[tid] = (int8_t)((nmt >> kSideShift) - 1);
+            }
+            const int q1 = q0 + GS;
+            if (tid < GS && q1 + tid < it.n) { nrc = rr[q1 + tid]; nmt = rmeta[q1 + tid]; }
+        } else {
+            for (int p = tid; p < gn; p += WG) {
+                const ExitRec rc = rr[q0 + p];
+                gx[p] = rc.x;
+                gy[p] = rc.y;
+                gY[p] = rc.Y;
+                gs[p] = rc.score;
+                gside[p] = (int8_t)((rmeta[q0 + p] >> kSideShift) - 1);
+            }
         }
         __syncthreads();
         // per (sub-batch, side) lists in path order: a wave per sub-batch, 64 paths per ballot
@@ -224,17 +247,17 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
             // side's exits of the sub-batch in path order, forms the chunk's RBF values of each
             // (rbf_side_chunk) and adds Y phi into one register sum per entry -- per entry the
             // order of the direct form
-            int cb[5];
-            cb[0] = 0;
-            for (int s = 0; s < 4; ++s) cb[s + 1] = cb[s] + (sd.msz[s] + kRbfChunk - 1) / kRbfChunk;
-            for (int t = tid; t < nb * cb[4]; t += WG) {
-                const int b = t / cb[4], r = t - b * cb[4];
-                int s = 0;
-                while (r >= cb[s + 1]) ++s;
-                const int m = sd.msz[s];
-                const int j0 = (r - cb[s]) * kRbfChunk;
+            const int cb1 = (ms0 + kRbfChunk - 1) / kRbfChunk;
+            const int cb2 = cb1 + (ms1 + kRbfChunk - 1) / kRbfChunk;
+            const int cb3 = cb2 + (ms2 + kRbfChunk - 1) / kRbfChunk;
+            const int cb4 = cb3 + (ms3 + kRbfChunk - 1) / kRbfChunk;
+            for (int t = tid; t < nb * cb4; t += WG) {
+                const int b = t / cb4, r = t - b * cb4;
+                const int s = r < cb1 ? 0 : r < cb2 ? 1 : r < cb3 ? 2 : 3;
+                const int m = msz_of(s);
+                const int j0 = (r - pick(s, 0, cb1, cb2, cb3)) * kRbfChunk;
                 const int len = (m - j0) < kRbfChunk ? (m - j0) : kRbfChunk;
-                const int e0 = sd.ent_off + eoff[s];
+                const int e0 = sd.ent_off + eoff_of(s);
                 const double *px = tb.entx + e0, *py = tb.enty + e0;
                 const double ic2 = tb.entic2[e0];
                 double sx = 0.0, sy = 0.0, ss = 1.0;
@@ -256,7 +279,7 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 }
 #pragma unroll
                 for (int k = 0; k < kRbfChunk; ++k)
-                    if (k < len) Phi[(eoff[s] + j0 + k) * kNBG + b] = acc[k];
+                    if (k < len) Phi[(eoff_of(s) + j0 + k) * kNBG + b] = acc[k];
             }
         } else
         for (int e = tid; e < ne; e += WG) {
@@ -300,10 +323,11 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 const int e = ec[q];
                 if (e < 0) continue;
                 const int se = tb.ents[sd.ent_off + e];
-                const int m = sd.msz[se];
-                const int el = e - eoff[se];
-                const double *A = tb.ipsi + sd.ipsi_off[se] + el;
-                const double2 *ph = (const double2 *)(Phi + eoff[se] * kNBG);
+                const int m = msz_of(se);
+                const int es = eoff_of(se);
+                const int el = e - es;
+                const double *A = tb.ipsi + ipsi_of(se) + el;
+                const double2 *ph = (const double2 *)(Phi + es * kNBG);
                 double H[kNBG];
 #pragma unroll
                 for (int b = 0; b < kNBG; ++b) H[b] = 0.0;

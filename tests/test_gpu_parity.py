@@ -119,6 +119,27 @@ def test_solve_knot_bit_exact(eng, knot, quirk):
     assert got["counters"]["exits"] == [int(v) for v in ref["cnt"]]
 
 
+@pytest.mark.parametrize("S,P", [(64, 1280), (48, 960), (8, 200)])
+def test_solve_knot_sub_batch_sizes_bit_exact(eng, S, P):
+    """Sub-batch sizes other than 32: with 48 or 64 paths a group of 8 sub-batches holds more
+    records than the reduction has threads (the records are then loaded in a loop, not one per
+    thread ahead of the group); with 8 the group is a quarter of the threads."""
+    from pdds_amd.engine import bvp, default_run
+    m = _product_mesh()
+    knot, n = 40, 2500
+    st = m.stencil(knot)
+    print("sub-batch %d, item %d" % (S, P), flush=True)
+    run = default_run(h=FIX_H, seed=11, sub_batch=S, item_paths=P)
+    got = eng.solve_knot(m, bvp("monegros"), run, knot, n)
+    pb = O.problem(O.PDE_MONEGROS)
+    paths = O.simulate(pb, O.run_params(FIX_H, 11, threads=8), m.x0[knot], m.y0[knot], st["rect"], knot, 0, n)
+    ref = O.reduce_batched(pb, paths, st, m.glob, _cshape(st, FIX_FAC), st["ipsi"], S, P, quirk=1)
+    assert np.array_equal(got["cols"], ref["cols"])
+    assert np.array_equal(got["G"], ref["G"])
+    assert np.array_equal(got["Gvar"], ref["Gvar"])
+    assert got["B"] == ref["B"] and got["BB"] == ref["BB"]
+
+
 @pytest.mark.parametrize("plain,lut_lds", [(0, 1), (0, 0), (1, 0)], ids=["tiled", "persistent", "plain"])
 def test_solve_knot_lut_bit_exact(eng, plain, lut_lds):
     """The assembly path kernels on the coefficient-LUT problem (the benchmark's kernels): the

@@ -97,7 +97,7 @@ PDDS_HD void rbf_side_chunk(const double *px, const double *py, int j0, double s
                             double x, double y, double inv_c2, const double *T, double Y, double acc[kRbfChunk]) {
     const double dx = px[j0] - x, dy = py[j0] - y;
     double p = dexp_tab_nonpos(-(dx * dx + dy * dy) * inv_c2, T);
-    double r = dexp(__builtin_fmin(-(2.0 * (dx * sx + dy * sy) + ss) * inv_c2, 700.0));
+    double r = dexp_fast(__builtin_fmin(-(2.0 * (dx * sx + dy * sy) + ss) * inv_c2, 700.0));
     acc[0] = __builtin_fma(Y, p, acc[0]);
 #pragma unroll
     for (int k = 1; k < kRbfChunk; ++k) {
@@ -222,7 +222,7 @@ __device__ __forceinline__ R draw_exponential(uint32_t st, uint32_t path, uint32
                                               uint32_t knot) {
     if constexpr (sizeof(R) == 8) {
         Philox4 w = philox4x32_10(st, path, knot, 0x80000000u, seed, stream);
-        return -dlog(u01_53(w.v[0], w.v[1]));
+        return -dlog_core(u01_53(w.v[0], w.v[1]), 0);
     } else {
         Philox4 w = philox4x32_10(st, path, knot, 0x80000000u, seed, stream);
         const float u = __builtin_fmaf((float)(w.v[0] >> 8), 5.9604644775390625e-08f, 2.98023223876953125e-08f);

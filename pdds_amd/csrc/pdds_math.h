@@ -69,6 +69,21 @@ PDDS_HD double dexp(double x) {
     return (p * pow2i(ki + 600)) * pow2i(-600);
 }
 
+// dexp(x) for |x| <= 708, where its result is normal (k in [-1021, 1022]): the same steps without
+// the special-case tests and scalings
+PDDS_HD double dexp_core(double x) {
+    const double k = __builtin_floor(x * kLog2e + 0.5);
+    double r = __builtin_fma(-k, kLn2Hi, x);
+    r = __builtin_fma(-k, kLn2Lo, r);
+    return exp_poly(r) * pow2i((int)k);
+}
+
+// dexp(x) bit for bit: the short path where the result is normal, a branch to dexp elsewhere
+PDDS_HD double dexp_fast(double x) {
+    if (__builtin_expect(__builtin_fabs(x) <= 708.0, 1)) return dexp_core(x);
+    return dexp(x);
+}
+
 // fma(p, r, c) with the constant c taken from a scalar register pair: one v_fma_f64 per Horner
 // step.  (Left to itself the compiler keeps the coefficients of a loop-resident polynomial in
 // vector registers and evaluates each step as a copy plus a two-address v_fmac_f64.)
@@ -192,17 +207,12 @@ PDDS_HD double dexpm1(double x) {
 // (e^x - 1) / x, 1 at x = 0 (the local-time weight of a Robin side)
 PDDS_HD double dexprel(double x) { return x == 0.0 ? 1.0 : dexpm1(x) / x; }
 
-PDDS_HD double dlog(double x) {
-    if (!(x > 0.0)) return (x == 0.0) ? -__builtin_huge_val() : __builtin_nan("");
-    if (x == __builtin_huge_val()) return x;
-    uint64_t b = f64_bits(x);
-    int e = (int)((b >> 52) & 0x7ff);
-    if (e == 0) {
-        x *= 18014398509481984.0;
-        b = f64_bits(x);
-        e = (int)((b >> 52) & 0x7ff) - 54;
-    }
-    e -= 1023;
+// dlog's steps for a positive, normal, finite x whose exponent is to be taken `sub` lower (0, or
+// -54 for a subnormal scaled by 2^54).  The Box-Muller uniforms of u01_53 lie in [2^-54, 1), so
+// their logs take this path directly, without dlog's special-case tests.
+PDDS_HD double dlog_core(double x, int sub) {
+    const uint64_t b = f64_bits(x);
+    int e = (int)((b >> 52) & 0x7ff) + sub - 1023;
     double m = f64_from((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
     if (m > 1.4142135623730951) { m *= 0.5; e += 1; }
     double s = (m - 1.0) / (m + 1.0);
@@ -223,6 +233,13 @@ PDDS_HD double dlog(double x) {
     double lm = s * p;
     double de = (double)e;
     return __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, lm));
+}
+
+PDDS_HD double dlog(double x) {
+    if (!(x > 0.0)) return (x == 0.0) ? -__builtin_huge_val() : __builtin_nan("");
+    if (x == __builtin_huge_val()) return x;
+    if (((f64_bits(x) >> 52) & 0x7ff) == 0) return dlog_core(x * 18014398509481984.0, -54);
+    return dlog_core(x, 0);
 }
 
 PDDS_HD void sincospi_kernel(double r, double &s, double &c) {
@@ -276,7 +293,7 @@ PDDS_HD double dtanh(double x) {
 // --------------------------------------------------------------- traits --
 template <class R> struct RM;
 template <> struct RM<double> {
-    static PDDS_HD double exp(double x) { return dexp(x); }
+    static PDDS_HD double exp(double x) { return dexp_fast(x); }
     static PDDS_HD double log(double x) { return dlog(x); }
     static PDDS_HD double sqrt(double x) { return __builtin_sqrt(x); }
     static PDDS_HD void sincospi(double x, double &s, double &c) { dsincospi(x, s, c); }
@@ -363,7 +380,7 @@ PDDS_HD void normals_f64(uint32_t seed, uint32_t stream, uint32_t knot, uint32_t
     Philox4 w = philox4x32_10((uint32_t)step, path, knot, (uint32_t)(step >> 32), seed, stream);
     double u1 = u01_53(w.v[0], w.v[1]);
     double u2 = u01_53(w.v[2], w.v[3]);
-    double r = __builtin_sqrt(-2.0 * dlog(u1));
+    double r = __builtin_sqrt(-2.0 * dlog_core(u1, 0));
     double s, c;
     dsincospi(2.0 * u2, s, c);
     n1 = r * c;

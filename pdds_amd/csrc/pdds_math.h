@@ -69,21 +69,6 @@ PDDS_HD double dexp(double x) {
     return (p * pow2i(ki + 600)) * pow2i(-600);
 }
 
-// dexp(x) for |x| <= 708, where its result is normal (k in [-1021, 1022]): the same steps without
-// the special-case tests and scalings
-PDDS_HD double dexp_core(double x) {
-    const double k = __builtin_floor(x * kLog2e + 0.5);
-    double r = __builtin_fma(-k, kLn2Hi, x);
-    r = __builtin_fma(-k, kLn2Lo, r);
-    return exp_poly(r) * pow2i((int)k);
-}
-
-// dexp(x) bit for bit: the short path where the result is normal, a branch to dexp elsewhere
-PDDS_HD double dexp_fast(double x) {
-    if (__builtin_expect(__builtin_fabs(x) <= 708.0, 1)) return dexp_core(x);
-    return dexp(x);
-}
-
 // fma(p, r, c) with the constant c taken from a scalar register pair: one v_fma_f64 per Horner
 // step.  (Left to itself the compiler keeps the coefficients of a loop-resident polynomial in
 // vector registers and evaluates each step as a copy plus a two-address v_fmac_f64.)
@@ -115,6 +100,23 @@ PDDS_HD double exp_poly_k(double r) {
     p = fma_k(p, r, 1.0);
     p = fma_k(p, r, 1.0);
     return p;
+}
+
+// dexp(x) for |x| <= 708, where its result is normal (k in [-1021, 1022]): the same steps without
+// the special-case tests and scalings.  Its polynomial keeps the two-instruction Horner steps:
+// with exp_poly_k as well, the LDS-tiled FP64 path kernel holds more coefficients in scalar
+// registers than it has, spills them and runs 3 % slower.
+PDDS_HD double dexp_core(double x) {
+    const double k = __builtin_floor(x * kLog2e + 0.5);
+    double r = __builtin_fma(-k, kLn2Hi, x);
+    r = __builtin_fma(-k, kLn2Lo, r);
+    return exp_poly(r) * pow2i((int)k);
+}
+
+// dexp(x) bit for bit: the short path where the result is normal, a branch to dexp elsewhere
+PDDS_HD double dexp_fast(double x) {
+    if (__builtin_expect(__builtin_fabs(x) <= 708.0, 1)) return dexp_core(x);
+    return dexp(x);
 }
 
 // dexp(x) for x <= 0, bit for bit, without branches: the value of dexp's own steps, with the
@@ -218,18 +220,19 @@ PDDS_HD double dlog_core(double x, int sub) {
     double s = (m - 1.0) / (m + 1.0);
     double s2 = s * s;
     // s^2 <= 0.0295: the atanh series terms past s^23 are below 1e-2 ulp
+    // (Horner steps through fma_k: one instruction each, the coefficient from scalar registers)
     double p = 0.08695652173913043;
-    p = __builtin_fma(p, s2, 0.09523809523809523);
-    p = __builtin_fma(p, s2, 0.10526315789473684);
-    p = __builtin_fma(p, s2, 0.11764705882352941);
-    p = __builtin_fma(p, s2, 0.13333333333333333);
-    p = __builtin_fma(p, s2, 0.15384615384615385);
-    p = __builtin_fma(p, s2, 0.18181818181818182);
-    p = __builtin_fma(p, s2, 0.2222222222222222);
-    p = __builtin_fma(p, s2, 0.2857142857142857);
-    p = __builtin_fma(p, s2, 0.4);
-    p = __builtin_fma(p, s2, 0.6666666666666666);
-    p = __builtin_fma(p, s2, 2.0);
+    p = fma_k(p, s2, 0.09523809523809523);
+    p = fma_k(p, s2, 0.10526315789473684);
+    p = fma_k(p, s2, 0.11764705882352941);
+    p = fma_k(p, s2, 0.13333333333333333);
+    p = fma_k(p, s2, 0.15384615384615385);
+    p = fma_k(p, s2, 0.18181818181818182);
+    p = fma_k(p, s2, 0.2222222222222222);
+    p = fma_k(p, s2, 0.2857142857142857);
+    p = fma_k(p, s2, 0.4);
+    p = fma_k(p, s2, 0.6666666666666666);
+    p = fma_k(p, s2, 2.0);
     double lm = s * p;
     double de = (double)e;
     return __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, lm));
@@ -247,24 +250,24 @@ PDDS_HD void sincospi_kernel(double r, double &s, double &c) {
     double y2 = y * y;
     // |y| <= pi/4: the Taylor terms past y^17 (sin) and y^18 (cos) are below 1e-3 ulp
     double ps = 2.8114572543455206e-15;
-    ps = __builtin_fma(ps, y2, -7.647163731819816e-13);
-    ps = __builtin_fma(ps, y2, 1.6059043836821613e-10);
-    ps = __builtin_fma(ps, y2, -2.505210838544172e-08);
-    ps = __builtin_fma(ps, y2, 2.7557319223985893e-06);
-    ps = __builtin_fma(ps, y2, -0.0001984126984126984);
-    ps = __builtin_fma(ps, y2, 0.008333333333333333);
-    ps = __builtin_fma(ps, y2, -0.16666666666666666);
+    ps = fma_k(ps, y2, -7.647163731819816e-13);
+    ps = fma_k(ps, y2, 1.6059043836821613e-10);
+    ps = fma_k(ps, y2, -2.505210838544172e-08);
+    ps = fma_k(ps, y2, 2.7557319223985893e-06);
+    ps = fma_k(ps, y2, -0.0001984126984126984);
+    ps = fma_k(ps, y2, 0.008333333333333333);
+    ps = fma_k(ps, y2, -0.16666666666666666);
     s = __builtin_fma(ps * y2, y, y);
     double pc = -1.5619206968586225e-16;
-    pc = __builtin_fma(pc, y2, 4.779477332387385e-14);
-    pc = __builtin_fma(pc, y2, -1.1470745597729725e-11);
-    pc = __builtin_fma(pc, y2, 2.08767569878681e-09);
-    pc = __builtin_fma(pc, y2, -2.755731922398589e-07);
-    pc = __builtin_fma(pc, y2, 2.48015873015873e-05);
-    pc = __builtin_fma(pc, y2, -0.001388888888888889);
-    pc = __builtin_fma(pc, y2, 0.041666666666666664);
-    pc = __builtin_fma(pc, y2, -0.5);
-    c = __builtin_fma(pc, y2, 1.0);
+    pc = fma_k(pc, y2, 4.779477332387385e-14);
+    pc = fma_k(pc, y2, -1.1470745597729725e-11);
+    pc = fma_k(pc, y2, 2.08767569878681e-09);
+    pc = fma_k(pc, y2, -2.755731922398589e-07);
+    pc = fma_k(pc, y2, 2.48015873015873e-05);
+    pc = fma_k(pc, y2, -0.001388888888888889);
+    pc = fma_k(pc, y2, 0.041666666666666664);
+    pc = fma_k(pc, y2, -0.5);
+    c = fma_k(pc, y2, 1.0);
 }
 
 // sin(pi x), cos(pi x); |x| < 2^52 assumed (path coordinates and uniforms)

@@ -97,6 +97,9 @@ struct pdds_server {
     int64_t n_submitted = 0, n_done = 0, n_rounds = 0, n_steps = 0;
     int64_t n_solver = 0;             // requests handed to the solver thread and not finished
     double busy_ms = 0;
+    // G values of a round, kept across rounds (the device thread's only): a round's knots can carry
+    // millions of values, and a fresh zeroed vector per round cost tens of milliseconds
+    std::vector<double> round_G;
     DevTables dom;                    // the domain's one-stencil table (point requests)
     bool dom_ready = false;
 };
@@ -243,9 +246,11 @@ bool run_round(pdds_server *s) {
     }
     pdds_status st = PDDS_OK;
     AsmOut o;
-    std::vector<double> Gr(std::max<int64_t>(n_vals, 1)), Gvr(std::max<int64_t>(n_vals, 1));
+    // (G variances are not part of a served result: none are copied back)
+    if ((int64_t)s->round_G.size() < std::max<int64_t>(n_vals, 1)) s->round_G.resize(std::max<int64_t>(n_vals, 1));
+    std::vector<double> &Gr = s->round_G;
     if (!knots.empty())
-        st = run_assembly(e, &s->cfg.problem, &s->cfg.run, s->cfg.domain, tabs, knots, n_vals, Gr.data(), Gvr.data(), o);
+        st = run_assembly(e, &s->cfg.problem, &s->cfg.run, s->cfg.domain, tabs, knots, n_vals, Gr.data(), nullptr, o);
     int64_t steps = 0;
     for (size_t i = 0; i < who.size() && !st; ++i) {
         SrvReq *r = who[i].r;

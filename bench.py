@@ -37,10 +37,14 @@ def progress(msg):
 
 
 def kernel_source_hash():
-    """SHA-256 (16 hex digits) of the engine's sources and build flags: a traffic figure measured
-    on another build is not reported for this one."""
+    """SHA-256 (16 hex digits) of the sources that decide the assembly kernels' memory traffic --
+    the kernels, their launch and tiling code, the stencil geometry -- and the build flags: a
+    traffic figure measured on another build of these is not reported for this one.  The serving
+    loop, the interface solver and the disc mesh do not take part in the measured assembly."""
     h = hashlib.sha256()
-    files = sorted(glob.glob(os.path.join(ROOT, "pdds_amd", "csrc", "*"))) + [os.path.join(ROOT, "pdds_amd", "Makefile")]
+    skip = {"server.cpp", "solver.hip", "solver.h", "disc_mesh.cpp"}
+    files = sorted(p for p in glob.glob(os.path.join(ROOT, "pdds_amd", "csrc", "*")) if os.path.basename(p) not in skip)
+    files += [os.path.join(ROOT, "pdds_amd", "Makefile")]
     for p in files:
         h.update(os.path.basename(p).encode())
         with open(p, "rb") as f:

@@ -134,6 +134,29 @@ def test_request_step_matches_direct_calls(eng):
     assert got_s["path_steps"] == out["counters"]["path_steps"]
 
 
+def test_served_solve_meets_the_error_target_with_summed_corners(eng):
+    """A 9x9 x 30-knot solve at the serving stream's settings (FP32 paths, 3000 paths per knot,
+    h = 1.2e-4 (2/9)^2): with the corner entries shared by two stencil sides summed, its interface
+    RMS error is below the 1e-3 target; with the reference's "=+" (only the last side's weight
+    kept) the same request carries a bias above it whatever the path count."""
+    from pdds_amd.engine import Mesh, Server, bvp, default_run
+    n_sub, n_node, n = 9, 30, 3000
+    h = 1.2e-4 * (2.0 / n_sub) ** 2
+    fac = _fac(n_sub, n_node, 0.8)
+    m = Mesh(DOMAIN[:2], DOMAIN[2:], n_sub, n_node, fac)
+    ex = bvp("helmholtz").u(m.px, m.py)
+    rms = {}
+    for quirk in (0, 1):
+        print("9x9 x 30 solve, quirk_last_side=%d" % quirk, flush=True)
+        srv = Server(eng, bvp("helmholtz"), default_run(h=h, seed=23, precision="fp32", quirk_last_side=quirk),
+                     DOMAIN)
+        got = srv.wait(srv.submit_solve(n_sub, n_node, fac, n, k2=0.0, stream=9, h=h), cap=m.n_knots)
+        srv.close()
+        assert got["status"] == 0 and got["n_knots"] == m.n_knots
+        rms[quirk] = float(np.sqrt(np.mean((got["u_knots"] - ex) ** 2)))
+    assert rms[0] < 1e-3 < rms[1], rms
+
+
 def test_budget_and_timing_fields(eng):
     from pdds_amd._capi import ERR_FULL, PddsError
     srv = _server(eng, hbm_budget=1 << 20)

@@ -69,6 +69,30 @@ def test_cfg2_knot_bit_exact(eng, cfg2, which, plain, lut_lds):
     assert got["counters"]["exits"] == [int(v) for v in ref["cnt"]]
 
 
+def test_cfg2_assembly_rows_equal_oracle(eng, cfg2):
+    """The benchmark's whole assembly -- 3409 Monte Carlo knots in one LDS-tiled launch, every
+    patch and queue of it, FP64 paths -- at 2000 paths per knot: the G rows, their variances and
+    the B entries of sampled knots (random ones, the largest segment stencil and a crossing knot)
+    equal the oracle's for those knots bit for bit."""
+    from pdds_amd.engine import default_run
+    mesh, prob, opb = cfg2
+    n = 2000
+    print("cfg2 assembly, FP64, %d paths per knot" % n, flush=True)
+    out = eng.assemble(mesh, prob, default_run(h=CFG2_H, seed=CFG2_SEED, sub_batch=32, item_paths=640), n)
+    rng = np.random.default_rng(3)
+    mc = np.flatnonzero(~mesh.on_boundary)
+    knots = [int(k) for k in rng.choice(mc, 5, replace=False)] + list(CFG2_KNOTS.values())
+    for k in knots:
+        st, cs = _cshape(mesh, k)
+        paths = O.simulate(opb, O.run_params(CFG2_H, CFG2_SEED, threads=8), mesh.x0[k], mesh.y0[k], st["rect"], k, 0, n)
+        ref = O.reduce_batched(opb, paths, st, mesh.glob, cs, st["ipsi"], 32, 640, quirk=1)
+        a, b = int(mesh.row_ptr[k]), int(mesh.row_ptr[k + 1])
+        assert np.array_equal(mesh.cols[a:b], ref["cols"]), k
+        assert np.array_equal(out["G"][a:b], ref["G"]), k
+        assert np.array_equal(out["Gvar"][a:b], ref["Gvar"]), k
+        assert out["B"][k] == ref["B"] and out["BB"][k] == ref["BB"], k
+
+
 @pytest.mark.parametrize("which", list(CFG2_KNOTS))
 def test_cfg2_knot_fp32_against_fp64_oracle(eng, cfg2, which):
     """FP32 path state (Kahan Z and xi) on the same Philox words as an FP64-stepped oracle: every

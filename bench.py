@@ -233,9 +233,9 @@ def summarize_traffic(pmc_dir, out_path):
         out[label] = rec
     if "path_kernel" in out:
         out["bytes_per_launch"] = out["path_kernel"]["bytes_per_launch"]
-    out["algorithmic_bytes_per_launch"] = 36 * 3409 * 100000
-    out["algorithmic_bytes_note"] = ("exit records: a 32-byte FP64 record (x, y, Y, score) and a 4-byte metadata "
-                                     "word per path, 3.409e8 paths")
+    out["algorithmic_bytes_per_launch"] = 32 * 3409 * 100000
+    out["algorithmic_bytes_note"] = ("exit records: a 32-byte FP64 record (x, y, Y, score) per path, 3.409e8 paths "
+                                     "(steps and step-cap stops are summed per knot, 16 bytes per knot)")
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))

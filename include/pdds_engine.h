@@ -328,7 +328,7 @@ pdds_status pdds_point_eval(pdds_engine *e, const pdds_problem *p, const pdds_ru
    rectangle's stencil table) is taken from one fixed array of PDDS_NOALLOC_SCRATCH_BYTES that
    each device holds from the library's load until the process ends; the call makes no hipMalloc
    or hipFree.  It returns PDDS_ERR_FULL, before any device work, when its buffers would exceed
-   that array: about 36 bytes per path (some 1.8e6 paths per call in total).  Every call on one
+   that array: about 32 bytes per path (some 2e6 paths per call in total).  Every call on one
    device shares the array, which keeps the last call's contents (what must start at zero is
    cleared by each call); overlapping calls on one device would race, so the library serialises
    them with a per-device lock (they wait for each other, also across engines). */

@@ -261,7 +261,8 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
     const int64_t max_steps = r->max_steps > 0 ? r->max_steps : (int64_t)1 << 26;
     if (r->item_paths > 0 && r->item_paths % S != 0)
         return fail(e, PDDS_ERR_INVALID, "item_paths must be a multiple of sub_batch");
-    if (max_steps > kStepMask) return fail(e, PDDS_ERR_INVALID, "max_steps must be below 2^27");
+    // a path's step count is a 32-bit register (the two-phase form adds a second cap of steps)
+    if (max_steps >= ((int64_t)1 << 27)) return fail(e, PDDS_ERR_INVALID, "max_steps must be below 2^27");
     out.S = S;
     out.wg = wg;
     out.kstats.assign(knots.size(), KnotStat{});
@@ -296,7 +297,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         out.wg = path_tiled_wg(r->precision, r->step);
     }
     // knots are processed in batches whose exit records fit the record budget
-    const int64_t rec_bytes = (int64_t)sizeof(ExitRec) + 4;
+    const int64_t rec_bytes = (int64_t)sizeof(ExitRec);
     const int64_t budget = e->record_budget / rec_bytes;
     void *p;
     double *dG, *dGv;
@@ -370,7 +371,9 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         if ((st = upload(e, B_ITEMS, items, &dit))) return st;
         Records rec;
         if ((st = dbuf(e, B_REC, (size_t)tot * sizeof(ExitRec), &p))) return st; rec.r = (ExitRec *)p;
-        if ((st = dbuf(e, B_RECMETA, (size_t)tot * 4, &p))) return st; rec.meta = (int32_t *)p;
+        if ((st = dbuf(e, B_KACC, (size_t)nk * 2 * sizeof(unsigned long long), &p))) return st;
+        rec.kacc = (unsigned long long *)p;
+        HIPCHK(launch_fill_u64(rec.kacc, 2 * (int64_t)nk, 0ULL, e->stream));
         ItemStat *dstats; double *dpart; KnotStat *dks; unsigned long long *dctr;
         if ((st = dbuf(e, B_STATS, items.size() * sizeof(ItemStat), &p))) return st; dstats = (ItemStat *)p;
         if ((st = dbuf(e, B_PART, (size_t)std::max<int64_t>(part, 1) * sizeof(double), &p))) return st; dpart = (double *)p;
@@ -453,6 +456,17 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         ra.stats = dstats; ra.partials = dpart; ra.ne_cap = ne_cap; ra.nnz_cap = nnz_cap; ra.S = S;
         ra.wref = dW;
         ra.side_rbf = geom == GeomRect ? 1 : 0;   // rectangle sides: the chunked recurrence
+        // exit sides: FP32 records carry them; FP64 exits are classified as write_exit does, on
+        // the global rectangle's centre and half-sides (rect_dist's) or the disc's radius
+        ra.side_mode = r->precision != PDDS_FP64 ? SIDE_BITS : geom == GeomDisc ? SIDE_DISC : SIDE_RECT;
+        if (geom == GeomDisc) {
+            ra.gc[0] = glob[2];
+        } else {
+            ra.gc[0] = 0.5 * (glob[0] + glob[2]);
+            ra.gc[1] = 0.5 * (glob[1] + glob[3]);
+            ra.gc[2] = glob[2] - ra.gc[0];
+            ra.gc[3] = glob[3] - ra.gc[1];
+        }
         const int64_t nit = (int64_t)items.size();
         for (int64_t base = 0; base < nit; base += (int64_t)1 << 30) {
             ra.item_base = base;
@@ -462,7 +476,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         }
         FinalizeArgs f;
         f.knots = dk; f.tables = dtab; f.items = dit; f.stats = dstats; f.partials = dpart;
-        f.n_knots = nk; f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks;
+        f.n_knots = nk; f.S = S; f.G = dG; f.Gvar = dGv; f.kstats = dks; f.kacc = rec.kacc;
         hipError_t lr = launch_finalize(nk, 256, e->stream, f);
         if (lr != hipSuccess) return fail(e, PDDS_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(lr));
         HIPCHK(hipEventRecord(e->ev[2], e->stream));
@@ -1334,10 +1348,11 @@ static pdds_status point_eval_locked(pdds_engine *e, const pdds_problem *p, cons
         paths += n_paths[i];
     }
     if (noalloc) {
-        // exit records, and per point its descriptor, statistics and work items (at most 17 of
-        // them, P >= N / 16, each with its descriptor and statistics), with room for alignment
-        const int64_t need = paths * (int64_t)(sizeof(ExitRec) + sizeof(int32_t)) +
-                             (int64_t)knots.size() * (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) +
+        // exit records, and per point its descriptor, statistics, step counters and work items (at
+        // most 17 of them, P >= N / 16, each with its descriptor and statistics), with room for
+        // alignment
+        const int64_t need = paths * (int64_t)sizeof(ExitRec) +
+                             (int64_t)knots.size() * (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) + 16 +
                                                                 17 * (sizeof(ItemDesc) + sizeof(ItemStat))) +
                              ((int64_t)1 << 16);
         if (need > (int64_t)PDDS_NOALLOC_SCRATCH_BYTES)

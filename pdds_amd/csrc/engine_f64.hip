@@ -111,20 +111,21 @@ hipError_t launch_paths_f64(int pde, int grid, int wg, hipStream_t s, const Path
 // One work-group per work item: per-side RBF moments per sub-batch, G_beta = -Psi^-T Phi per
 // CSR column, sums in fixed order (sub-batch -> item), B sums per sub-batch.  A group of kNBG
 // sub-batches is staged in LDS with, per (sub-batch, side), the list of its paths that exit
-// through that side in path order (built with wave ballots).  A thread then owns one stencil
+// through that side in path order (built with wave ballots, each exit's side read from its
+// record's Y bits or found from its exit point, and counted per side).  A thread then owns one stencil
 // entry e and, for each sub-batch b, adds Y phi_e(X) over the list of e's side in path order
 // (the order of the per-exit loop it replaces), four exits per iteration so that four exp chains
 // are in flight; Phi is kept [entry][sub-batch] so that the Psi^-T pass reads a row of kNBG
 // values with two 16-byte LDS reads per pair of sub-batches.
 // EQ: the launch carries reference values (ReduceArgs::wref) and also sums the knot equations
-template <bool EQ>
+// SIDE (ReduceSide): how the exit side of a record is found
+template <bool EQ, int SIDE>
 __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(4))) reduce_kernel(ReduceArgs a) {
     extern __shared__ __align__(16) double smem[];
     __shared__ double s_bsb[kNBG], s_bbsb[kNBG];
     __shared__ double s_wsum[(kReduceWG / 64) * kNBG];
     __shared__ double s_exp2[64];
     __shared__ int s_lcnt[kNBG * 4];
-    __shared__ unsigned long long s_steps;
     __shared__ unsigned int s_cnt[8];
     const int tid = threadIdx.x;
     const int WG = blockDim.x;
@@ -136,7 +137,6 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     const StencilDesc &sd = tb.sten[kd.stencil];
     const int64_t g0 = kd.path_off + it.p_begin;
     const ExitRec *rr = a.rec.r + g0;
-    const int32_t *rmeta = a.rec.meta + g0;
     const int S = a.S;
     const int GS = kNBG * S;
     double *Phi = smem;                                     // [ne_cap][kNBG]
@@ -147,27 +147,21 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     double *gY = gy + GS;
     double *gs = gY + GS;
     uint16_t *lst = (uint16_t *)(gs + GS);
-    int8_t *gside = (int8_t *)(lst + 4 * GS);
-    if (tid == 0) {
-        s_steps = 0ULL;
+    if (tid == 0)
         for (int q = 0; q < 8; ++q) s_cnt[q] = 0u;
-    }
     if (tid < 64) s_exp2[tid] = kExp2TabDev[tid];
     __syncthreads();
-    {
-        unsigned long long my = 0ULL;
-        unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int p = tid; p < it.n; p += WG) {
-            const int32_t mt = rmeta[p];
-            my += (unsigned long long)(mt & kStepMask);
-            if (mt & kCappedBit) cnt[6]++;
-            const int sd0 = (mt >> kSideShift) - 1;
-            cnt[sd0 < 0 ? 5 : sd0]++;
+    // exit side of a record (stencil side 0..3, 4 global boundary, -1 none) and its Y
+    auto side_of = [&](double x, double y, double Yr, double &Y) -> int {
+        if constexpr (SIDE == SIDE_BITS) {
+            const uint64_t yb = __builtin_bit_cast(uint64_t, Yr);
+            Y = __builtin_bit_cast(double, yb & ~kSideBitsMask);
+            return (int)(yb & kSideBitsMask) - 1;
         }
-        atomicAdd(&s_steps, my);
-        for (int q = 0; q < 7; ++q)
-            if (cnt[q]) atomicAdd(&s_cnt[q], cnt[q]);
-    }
+        Y = Yr;
+        if constexpr (SIDE == SIDE_DISC) return classify_sector(sd.geo, a.gc[0], x, y);
+        else return classify_coord_c(sd.rect, a.gc, x, y);
+    };
     const int ne = sd.msz[0] + sd.msz[1] + sd.msz[2] + sd.msz[3];
     const int nnz = sd.csr_nnz;
     const int nsb = (it.n + S - 1) / S;
@@ -181,45 +175,35 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
     auto ipsi_of = [&](int s) { return pick(s, io0, io1, io2, io3); };
     for (int c = tid; c < nnz; c += WG) { accS[c] = 0.0; accQ[c] = 0.0; }
     double Bs = 0.0, BBs = 0.0, Rs = 0.0, RQs = 0.0;
-    // a group's records fit one per thread (GS <= WG): each thread loads its record of the next
-    // group while the current group is reduced, so the loads' latency overlaps the work
-    const bool one = GS <= WG;
-    ExitRec nrc{0.0, 0.0, 0.0, 0.0};
-    int32_t nmt = 0;
-    if (one && tid < GS && tid < it.n) { nrc = rr[tid]; nmt = rmeta[tid]; }
+    // (holding the next group's records in registers while a group is reduced, to overlap their
+    // loads with the work, takes 8 more registers than the four-wave budget leaves: with the
+    // exit side found below it spills and runs the reduction 35 % slower)
     for (int g = 0; g < nsb; g += kNBG) {
         const int nb = (nsb - g) < kNBG ? (nsb - g) : kNBG;
         const int q0 = g * S;                                   // first path of the group
         const int gn = (it.n - q0) < nb * S ? (it.n - q0) : nb * S;
-        if (one) {
-            if (tid < gn) {
-                gx[tid] = nrc.x;
-                gy[tid] = nrc.y;
-                gY[tid] = nrc.Y;
-                gs[tid] = nrc.score;
-                gside[tid] = (int8_t)((nmt >> kSideShift) - 1);
-            }
-            const int q1 = q0 + GS;
-            if (tid < GS && q1 + tid < it.n) { nrc = rr[q1 + tid]; nmt = rmeta[q1 + tid]; }
-        } else {
-            for (int p = tid; p < gn; p += WG) {
-                const ExitRec rc = rr[q0 + p];
-                gx[p] = rc.x;
-                gy[p] = rc.y;
-                gY[p] = rc.Y;
-                gs[p] = rc.score;
-                gside[p] = (int8_t)((rmeta[q0 + p] >> kSideShift) - 1);
-            }
+        for (int p = tid; p < gn; p += WG) {
+            const ExitRec rc = rr[q0 + p];
+            gx[p] = rc.x;
+            gy[p] = rc.y;
+            gY[p] = rc.Y;
+            gs[p] = rc.score;
         }
         __syncthreads();
         // per (sub-batch, side) lists in path order: a wave per sub-batch, 64 paths per ballot
         for (int b = wave; b < nb; b += nwaves) {
             const int p0 = b * S, p1 = (p0 + S) < gn ? (p0 + S) : gn;
-            int cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0;
+            int cnt0 = 0, cnt1 = 0, cnt2 = 0, cnt3 = 0, cnt4 = 0;
             const unsigned long long below = (1ULL << lane) - 1ULL;
             for (int c0 = p0; c0 < p1; c0 += 64) {
                 const int p = c0 + lane;
-                const int sdp = p < p1 ? (int)gside[p] : -1;
+                // the exit side (FP32 records' Y loses its side bits here)
+                int sdp = -2;
+                if (p < p1) {
+                    double Y;
+                    sdp = side_of(gx[p], gy[p], gY[p], Y);
+                    if constexpr (SIDE == SIDE_BITS) gY[p] = Y;
+                }
                 uint16_t *Lb = lst + (b * 4) * S;
                 const unsigned long long m0 = __ballot(sdp == 0), m1 = __ballot(sdp == 1);
                 const unsigned long long m2 = __ballot(sdp == 2), m3 = __ballot(sdp == 3);
@@ -228,10 +212,16 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
                 if (sdp == 2) Lb[2 * S + cnt2 + __popcll(m2 & below)] = (uint16_t)(p - p0);
                 if (sdp == 3) Lb[3 * S + cnt3 + __popcll(m3 & below)] = (uint16_t)(p - p0);
                 cnt0 += __popcll(m0); cnt1 += __popcll(m1); cnt2 += __popcll(m2); cnt3 += __popcll(m3);
+                cnt4 += __popcll(__ballot(sdp == 4));
             }
             if (lane == 0) {
                 s_lcnt[b * 4 + 0] = cnt0; s_lcnt[b * 4 + 1] = cnt1;
                 s_lcnt[b * 4 + 2] = cnt2; s_lcnt[b * 4 + 3] = cnt3;
+                // the item's exits per side
+                atomicAdd(&s_cnt[0], (unsigned)cnt0); atomicAdd(&s_cnt[1], (unsigned)cnt1);
+                atomicAdd(&s_cnt[2], (unsigned)cnt2); atomicAdd(&s_cnt[3], (unsigned)cnt3);
+                atomicAdd(&s_cnt[4], (unsigned)cnt4);
+                atomicAdd(&s_cnt[5], (unsigned)((p1 - p0) - (cnt0 + cnt1 + cnt2 + cnt3 + cnt4)));
             } else if (lane == 1) {
                 // B sums of the sub-batch in path order
                 double bs = 0.0, bb = 0.0;
@@ -409,25 +399,33 @@ __global__ void __launch_bounds__(kReduceWG) __attribute__((amdgpu_waves_per_eu(
         st.BBs = BBs;
         st.Rs = Rs;
         st.RQs = RQs;
-        st.steps = s_steps;
         for (int q = 0; q < 6; ++q) st.cnt[q] = s_cnt[q];
-        st.capped = s_cnt[6];
-        st.pad = 0;
         a.stats[item] = st;
     }
 }
 
-hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
+template <bool EQ, int SIDE> static hipError_t launch_reduce_t(int grid, int wg, size_t shmem, hipStream_t s,
+                                                            const ReduceArgs &a) {
     if (shmem > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(a.wref ? (const void *)reduce_kernel<true> : (const void *)reduce_kernel<false>,
+        hipError_t e = hipFuncSetAttribute((const void *)reduce_kernel<EQ, SIDE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
         if (e != hipSuccess) return e;
     }
-    if (a.wref)
-        hipLaunchKernelGGL(reduce_kernel<true>, dim3(grid), dim3(wg), shmem, s, a);
-    else
-        hipLaunchKernelGGL(reduce_kernel<false>, dim3(grid), dim3(wg), shmem, s, a);
+    hipLaunchKernelGGL((reduce_kernel<EQ, SIDE>), dim3(grid), dim3(wg), shmem, s, a);
     return hipGetLastError();
+}
+
+template <bool EQ> static hipError_t launch_reduce_eq(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
+    switch (a.side_mode) {
+        case SIDE_BITS: return launch_reduce_t<EQ, SIDE_BITS>(grid, wg, shmem, s, a);
+        case SIDE_RECT: return launch_reduce_t<EQ, SIDE_RECT>(grid, wg, shmem, s, a);
+        case SIDE_DISC: return launch_reduce_t<EQ, SIDE_DISC>(grid, wg, shmem, s, a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_reduce(int grid, int wg, size_t shmem, hipStream_t s, const ReduceArgs &a) {
+    return a.wref ? launch_reduce_eq<true>(grid, wg, shmem, s, a) : launch_reduce_eq<false>(grid, wg, shmem, s, a);
 }
 
 // One block per knot: sum the knot's items in item order.
@@ -454,8 +452,8 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
     if (threadIdx.x == 0) {
         double Bs = 0.0, BBs = 0.0, Rs = 0.0, RQs = 0.0;
         KnotStat ks;
-        ks.steps = 0;
-        ks.capped = 0;
+        ks.steps = f.kacc[2 * (int64_t)k];
+        ks.capped = f.kacc[2 * (int64_t)k + 1];
         for (int q = 0; q < 6; ++q) ks.cnt[q] = 0;
         for (int64_t w = w0; w < w1; ++w) {
             const ItemStat st = f.stats[w];
@@ -463,8 +461,6 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
             BBs = BBs + st.BBs;
             Rs = Rs + st.Rs;
             RQs = RQs + st.RQs;
-            ks.steps += st.steps;
-            ks.capped += st.capped;
             for (int q = 0; q < 6; ++q) ks.cnt[q] += st.cnt[q];
         }
         ks.B = Bs / N;

@@ -89,7 +89,7 @@ struct pdds_mesh {
 
 enum BufSlot {
     B_KNOTS, B_STEN, B_ENTX, B_ENTY, B_ENTS, B_E1, B_E2, B_IPSI, B_ITEMS, B_STATS, B_PART, B_REC,
-    B_RECMETA, B_COUNTER, B_PATCH, B_TABLES, B_G, B_GVAR, B_KSTAT, B_WREF, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
+    B_KACC, B_COUNTER, B_PATCH, B_TABLES, B_G, B_GVAR, B_KSTAT, B_WREF, B_P0, B_P1, B_P2, B_P3, B_P4, B_P5, B_P6,
     B_P7, B_P8, B_P9, B_P10, B_P11, B_NSLOTS
 };
 

@@ -10,8 +10,9 @@
 //    (Solvers/SolveLoop.cuh:59-74, solveCUDA.cuh:118-168).  Each path is stepped
 //    by the control-variate Euler-Maruyama scheme (Integrators/FKACIntegrator.hpp:46-57),
 //    stopped by the Gobet-shifted test (Integrators/FKACIntegrator.hpp:168-186),
-//    and its exit record (projection, Y, score, side, steps) is written at its
-//    global path index.  The plain form launches one lane per path, no refill.
+//    and its exit record (projection, Y, score) is written at its global path
+//    index; steps are summed per knot.  The plain form launches one lane per path,
+//    no refill.
 //  * reduce_kernel: one work-group per work item of P paths of one knot.  Per
 //    sub-batch of S paths it forms the per-side RBF moments Phi_j = sum Y phi_j(X)
 //    (m exps per exit, where Meshes/stencil.cpp:244-307 spends m^2), then
@@ -45,6 +46,21 @@ PDDS_HD int classify_coord(const double rect[4], const double glob[4], double x,
     double ex, ey, nx, ny;
     int kind;
     if (rect_dist<double>(glob, x, y, ex, ey, nx, ny, kind) >= -1e-8) return 4;
+    if (__builtin_fabs(y - rect[1]) < 1e-8) return 1;
+    if (__builtin_fabs(y - rect[3]) < 1e-8) return 0;
+    if (__builtin_fabs(x - rect[0]) < 1e-8) return 3;
+    if (__builtin_fabs(x - rect[2]) < 1e-8) return 2;
+    return -1;
+}
+
+// classify_coord with the global rectangle given by its centre and half-sides gc = {cx, cy, hx, hy}
+// (rect_dist's own cx = 0.5 (x0 + x1), hx = x1 - cx): rect_dist's value is max(|x - cx| - hx,
+// |y - cy| - hy) inside the rectangle (its tie rule picks between equal values) and positive
+// outside, where that maximum is positive too, so for finite coordinates the global test is the
+// same comparison without the projection.
+PDDS_HD int classify_coord_c(const double rect[4], const double gc[4], double x, double y) {
+    const double ax = __builtin_fabs(x - gc[0]) - gc[2], ay = __builtin_fabs(y - gc[1]) - gc[3];
+    if (__builtin_fmax(ax, ay) >= -1e-8) return 4;
     if (__builtin_fabs(y - rect[1]) < 1e-8) return 1;
     if (__builtin_fabs(y - rect[3]) < 1e-8) return 0;
     if (__builtin_fabs(x - rect[0]) < 1e-8) return 3;
@@ -290,13 +306,12 @@ __device__ __forceinline__ int find_knot(const KnotDesc *knots, int n_knots, int
     return lo;
 }
 
-// Exit record of a stopped path at batch slot g: the exit projection, Y, the score
+// Exit record of a stopped path at batch slot g: the exit projection, Y and the score
 // (Z + xi, plus Y g(X) for a global-boundary exit, Reduce_Analytic
-// Solvers/EMFKACSolver.hpp:559-566), the exit side and the step count (kCappedBit: stopped by
-// the step cap while still inside).
+// Solvers/EMFKACSolver.hpp:559-566); FP32 paths also put their exit side in Y's low bits.
 template <class R, int PDE, int GEOM>
 __device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, const PathState<R> &s,
-                                           const R *dom, int gmask, R ex, R ey, int kind, bool capped_inside) {
+                                           const R *dom, int gmask, R ex, R ey, int kind) {
     int side;
     if constexpr (sizeof(R) == 8) {
         if constexpr (GEOM == GeomDisc) side = classify_sector(dom, a.glob[2], (double)ex, (double)ey);
@@ -315,10 +330,36 @@ __device__ __forceinline__ void write_exit(const PathKernelArgs &a, uint32_t g, 
     rc.x = (double)ex;
     rc.y = (double)ey;
     rc.Y = (double)s.Y;
+    if constexpr (sizeof(R) == 4)
+        rc.Y = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, rc.Y) | (uint64_t)(side + 1));
     rc.score = sc;
     a.rec.r[g] = rc;
-    a.rec.meta[g] = (int32_t)s.st | (capped_inside ? kCappedBit : 0) | ((side + 1) << kSideShift);
 }
+
+// Steps and step-cap stops of stopped paths of one knot, summed in a lane's registers and added
+// to the knot's counters (Records::kacc) when the lane moves to another knot and when it ends.
+struct KnotSteps {
+    int k;                    // knot of the sums
+    unsigned long long st;    // steps
+    uint32_t cap;             // paths stopped by the step cap while inside
+
+    __device__ __forceinline__ void init() { k = 0; st = 0ULL; cap = 0u; }
+    __device__ __forceinline__ void flush(const PathKernelArgs &a) {
+        if (st) {
+            atomicAdd(a.rec.kacc + 2 * (int64_t)k, st);
+            if (cap) atomicAdd(a.rec.kacc + 2 * (int64_t)k + 1, (unsigned long long)cap);
+            st = 0ULL;
+            cap = 0u;
+        }
+    }
+    // the lane takes a path of knot kn
+    __device__ __forceinline__ void enter(const PathKernelArgs &a, int kn) {
+        if (kn != k) {
+            flush(a);
+            k = kn;
+        }
+    }
+};
 
 // Distance to a stencil's stopping domain: its rectangle, or its annular sector on a disc mesh.
 template <class R, int GEOM>
@@ -358,15 +399,18 @@ struct Walker {
     // local time
     int rmask, rk;
     R rd, rnx, rny, lt;
+    KnotSteps ks;          // steps of the lane's stopped paths not yet added to their knot's counters
 
     __device__ __forceinline__ void init() {
         pn1 = 0; pn2 = 0; kid = 0; kstream = 0; gmask = 0; g = 0; k2 = 0; hk = 0; sqhk = 0;
         active = false; fresh = false;
         rmask = 0; rk = 0; rd = 0; rnx = 0; rny = 0; lt = 0;
+        ks.init();
     }
 
     // take path gg of the batch; k is its knot
     __device__ __forceinline__ void start(const PathKernelArgs &a, int64_t gg, int k) {
+        ks.enter(a, k);
         const KnotDesc &kd = a.knots[k];
         const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
         const double *dom = domain_of<GEOM>(sd);
@@ -467,7 +511,9 @@ struct Walker {
                 R tx, ty;
                 (void)rect_dist<R>(rect, s.x, s.y, ex, ey, tx, ty, kind);
             }
-            write_exit<R, PDE, GEOM>(a, g, s, rect, gmask, ex, ey, kind, capped && go);
+            write_exit<R, PDE, GEOM>(a, g, s, rect, gmask, ex, ey, kind);
+            ks.st += s.st;
+            ks.cap += capped && go ? 1u : 0u;
             active = false;
         }
         fresh = false;
@@ -562,6 +608,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(path_w
         }
         w.template step<false, false>(a, none, it);
     }
+    w.ks.flush(a);
 }
 
 // ----------------------------------------------------------------------------
@@ -581,7 +628,8 @@ template <class R, int PDE, int GEOM, int FAM, bool REFL = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_waves<R, FAM>()))) plain_kernel(PathKernelArgs a) {
     const int64_t gg = a.base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gg >= a.total) return;
-    const KnotDesc &kd = a.knots[find_knot(a.knots, a.n_knots, gg)];
+    const int kn = find_knot(a.knots, a.n_knots, gg);
+    const KnotDesc &kd = a.knots[kn];
     const StencilDesc &sd = a.tables[kd.table].sten[kd.stencil];
     const LutTile none{nullptr, 0, 0, 0, 0};
     const R h = (R)kd.h, sqrth = (R)kd.sqrth, beta = (R)a.beta, k2 = (R)kd.k2;
@@ -636,7 +684,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
         go = rnow || d < beta * nrm * sqrth;
         capped = (int64_t)s.st >= a.max_steps;
     } while (go && !capped);
-    write_exit<R, PDE, GEOM>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind, capped && go);
+    write_exit<R, PDE, GEOM>(a, (uint32_t)gg, s, rect, sd.gmask, ex, ey, kind);
+    // steps and step-cap stops into the knot's counters: one wave sum and one atomic per counter
+    // when all 64 lanes hold paths of one knot, else an atomic per lane
+    const bool cap = capped && go;
+    const int k0 = __builtin_amdgcn_readfirstlane(kn);
+    if (__ballot(1) == ~0ULL && __all(kn == k0)) {
+        unsigned long long v = s.st;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        const unsigned long long nc = (unsigned long long)__popcll(__ballot(cap));
+        if (__lane_id() == 0) {
+            atomicAdd(a.rec.kacc + 2 * (int64_t)kn, v);
+            if (nc) atomicAdd(a.rec.kacc + 2 * (int64_t)kn + 1, nc);
+        }
+    } else {
+        atomicAdd(a.rec.kacc + 2 * (int64_t)kn, (unsigned long long)s.st);
+        if (cap) atomicAdd(a.rec.kacc + 2 * (int64_t)kn + 1, 1ULL);
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -718,12 +782,13 @@ path_kernel_tiled(PathKernelArgs a) {
         }
         __syncthreads();   // every wave is done with this tile
     }
+    w.ks.flush(a);
 }
 
 // ----------------------------------------------------------------------------
 // Reduction kernel: one work-group per work item
 // ----------------------------------------------------------------------------
-template <bool EQ> __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a);
+template <bool EQ, int SIDE> __global__ void __launch_bounds__(kReduceWG) reduce_kernel(ReduceArgs a);
 
 // One block per knot: sum the knot's items in item order.
 __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f);

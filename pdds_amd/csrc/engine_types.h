@@ -10,20 +10,17 @@ constexpr int kReduceWG = 256;            // reduction work-group
 constexpr int kRbfChunk = 16;             // side entries per direct exponential (rectangle sides)
 
 // LDS of the reduction kernel: Phi [ne_cap][kNBG], accS/accQ [nnz_cap] (FP64), the group's
-// records x, y, Y and score [kNBG*S] (FP64), their sides [kNBG*S] (int8) and per-(sub-batch,
-// side) path lists [kNBG][4][S] (uint16)
+// records x, y, Y and score [kNBG*S] (FP64) and per-(sub-batch, side) path lists [kNBG][4][S]
+// (uint16)
 inline size_t reduce_lds_bytes(int ne_cap, int nnz_cap, int S) {
     const size_t g = (size_t)kNBG * S;
     size_t b = ((size_t)kNBG * ne_cap + 2 * (size_t)nnz_cap + 4 * g) * sizeof(double);
-    b += g * 4 * sizeof(uint16_t) + g;
+    b += g * 4 * sizeof(uint16_t);
     return (b + 15) & ~(size_t)15;
 }
-// Exit-record metadata word: the step count in bits 0..26, bit 27 set when the path was
-// stopped by the step cap, the exit side + 1 (0 = none, 1..6 = sides -1..5) in bits 28..30.
-constexpr int32_t kStepBits = 27;
-constexpr int32_t kStepMask = (1 << kStepBits) - 1;
-constexpr int32_t kCappedBit = 1 << kStepBits;
-constexpr int32_t kSideShift = 28;
+// FP32 exit records carry the exit side + 1 (0 = none, 1..5 = sides -1..4) in the low bits of
+// Y: a float widened to double leaves the low 29 bits of its mantissa zero.
+constexpr uint64_t kSideBitsMask = 7ULL;
 constexpr int kMaxQueues = 8;              // path queues: one per XCD of an MI355X (8 XCDs x 32 CUs)
 
 // One Monte Carlo knot of an assembly batch.
@@ -78,10 +75,7 @@ struct ItemDesc {
 struct ItemStat {
     double Bs, BBs;
     double Rs, RQs;         // equation residual sums (ReduceArgs::wref set): sum R_b, sum R_b^2 / n_b
-    unsigned long long steps;
-    uint32_t cnt[6];
-    uint32_t capped;
-    uint32_t pad;
+    uint32_t cnt[6];        // exits per side 0..4, and paths stopped inside (no side)
 };
 
 struct KnotStat {
@@ -94,13 +88,17 @@ struct KnotStat {
 
 // Per-path exit records, indexed by global path index: the FP64 exit values as one aligned
 // 32-byte record (a lane stores it with two 16-byte writes when its path stops, so a record
-// reaches memory as one 32-byte piece), and the metadata word (steps, cap flag, side).
+// reaches memory as one 32-byte piece).  The exit side is not stored for FP64 paths: the
+// reduction classifies the exit point again, with the same test on the same doubles.  Step
+// counts are not stored per path: a lane sums the steps and step-cap stops of its stopped paths
+// in registers and adds them to its knot's counters kacc[2k] (steps) and kacc[2k + 1] (capped)
+// when it moves to a path of another knot and when it ends.
 struct alignas(32) ExitRec {
     double x, y, Y, score;
 };
 struct Records {
     ExitRec *r;
-    int32_t *meta;
+    unsigned long long *kacc;   // 2 per knot of the launch, zero before the path kernel
 };
 
 // A patch: knots whose LUT tile (the plane-a texels their stencil rectangle can touch) is the
@@ -151,7 +149,13 @@ struct ReduceArgs {
     // 1: every stencil side is straight and uniformly spaced with one Gaussian width (rectangle
     // meshes), and the RBF values of a side come from rbf_side_chunk
     int32_t side_rbf;
+    // exit sides (ReduceSide): from the low bits of the records' Y (FP32 paths), or by
+    // classifying the exit point as write_exit does for FP64 paths, on the global
+    // rectangle of centre and half-sides gc (classify_coord_c) or the disc of radius gc[0]
+    int32_t side_mode;
+    double gc[4];
 };
+enum ReduceSide { SIDE_BITS = 0, SIDE_RECT = 1, SIDE_DISC = 2 };
 
 struct FinalizeArgs {
     const KnotDesc *knots;
@@ -163,6 +167,7 @@ struct FinalizeArgs {
     int32_t S;
     double *G, *Gvar;
     KnotStat *kstats;
+    const unsigned long long *kacc;   // steps and capped paths per knot (Records::kacc)
 };
 
 struct PathsArgs {

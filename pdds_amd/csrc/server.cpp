@@ -31,11 +31,13 @@ namespace {
 
 enum ReqState { RS_PREP = 0, RS_READY, RS_ADMITTED, RS_DONE };
 
-// device bytes per path of a round: its exit record and metadata word
-constexpr int64_t kRecordBytes = (int64_t)sizeof(ExitRec) + (int64_t)sizeof(int32_t);
+// device bytes per path of a round: its exit record
+constexpr int64_t kRecordBytes = (int64_t)sizeof(ExitRec);
 // device bytes per work item of a knot besides its records and partials: the item's descriptor
-// and statistics, and (counted per item, an over-estimate) the knot's descriptor and statistics
-constexpr int64_t kKnotBytes = (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) + sizeof(ItemDesc) + sizeof(ItemStat));
+// and statistics, and (counted per item, an over-estimate) the knot's descriptor, statistics and
+// step counters
+constexpr int64_t kKnotBytes = (int64_t)(sizeof(KnotDesc) + sizeof(KnotStat) + 2 * sizeof(unsigned long long) +
+                                         sizeof(ItemDesc) + sizeof(ItemStat));
 
 struct SrvReq {
     uint64_t ticket = 0;

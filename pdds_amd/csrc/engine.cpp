@@ -294,7 +294,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
             rank[i] = it->second;
         }
         std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return rank[a] < rank[b]; });
-        out.wg = path_tiled_wg(r->precision, r->step);
+        out.wg = path_tiled_wg(r->precision, r->step, geom);
     }
     // knots are processed in batches whose exit records fit the record budget
     const int64_t rec_bytes = (int64_t)sizeof(ExitRec);
@@ -428,7 +428,7 @@ pdds_status run_assembly(pdds_engine *e, const pdds_problem *prob, const pdds_ru
         pa.n_queues = persistent ? std::min(kMaxQueues, grid) : 1;
         pa.patches = dpatch;
         pa.n_patches = (int32_t)patches.size();
-        const int twg = path_tiled_wg(r->precision, r->step);
+        const int twg = path_tiled_wg(r->precision, r->step, geom);
         if (tiled) {
             grid = r->grid_wgs > 0 ? r->grid_wgs
                                    : (int)std::max<int64_t>(1, std::min<int64_t>(e->n_cu, (tot + 8 * twg - 1) / (8 * twg)));

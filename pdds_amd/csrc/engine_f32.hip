@@ -59,7 +59,7 @@ static hipError_t path_tiled_f32_f(int grid, size_t lds, hipStream_t s, const Pa
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<float, FAM>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<float, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<float, FAM, GEOM>()), lds, s, a);
     return hipGetLastError();
 }
 

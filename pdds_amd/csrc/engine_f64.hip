@@ -57,9 +57,14 @@ hipError_t launch_path_f64(int pde, int geom, bool persistent, bool mixed, int g
     }
 }
 
-int path_tiled_wg(int precision, int step) {
-    if (step == STEP_CV) return precision == PDDS_FP64 ? tiled_wg<double, STEP_CV>() : tiled_wg<float, STEP_CV>();
-    return precision == PDDS_FP64 ? tiled_wg<double, STEP_ANY>() : tiled_wg<float, STEP_ANY>();
+template <int GEOM> static int path_tiled_wg_g(int precision, int step) {
+    if (step == STEP_CV)
+        return precision == PDDS_FP64 ? tiled_wg<double, STEP_CV, GEOM>() : tiled_wg<float, STEP_CV, GEOM>();
+    return precision == PDDS_FP64 ? tiled_wg<double, STEP_ANY, GEOM>() : tiled_wg<float, STEP_ANY, GEOM>();
+}
+
+int path_tiled_wg(int precision, int step, int geom) {
+    return geom == GeomDisc ? path_tiled_wg_g<GeomDisc>(precision, step) : path_tiled_wg_g<GeomRect>(precision, step);
 }
 
 template <bool MIXED, int GEOM, int FAM>
@@ -69,7 +74,7 @@ static hipError_t path_tiled_f64_f(int grid, size_t lds, hipStream_t s, const Pa
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<double, FAM>()), lds, s, a);
+    hipLaunchKernelGGL((path_kernel_tiled<double, MIXED, GEOM, FAM>), dim3(grid), dim3(tiled_wg<double, FAM, GEOM>()), lds, s, a);
     return hipGetLastError();
 }
 

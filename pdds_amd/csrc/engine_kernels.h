@@ -712,8 +712,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_
 // go to the vector L1.
 // ----------------------------------------------------------------------------
 // one work-group per CU holding every wave the register budget allows (the run-time step-family
-// variants one wave less, as in path_kernel)
-template <class R, int FAM> constexpr int tiled_wg() { return 64 * 4 * path_waves_f<R, PDE_VARCOEF_LUT, FAM>(); }
+// variants one wave less, as in path_kernel, and so the FP32 annular-sector kernels, whose
+// sector distance needs more registers than three waves per SIMD leave)
+template <class R, int FAM, int GEOM> constexpr int tiled_waves() {
+    return path_waves_f<R, PDE_VARCOEF_LUT, FAM>() -
+           (sizeof(R) == 4 && GEOM == GeomDisc && path_waves_f<R, PDE_VARCOEF_LUT, FAM>() > 2 ? 1 : 0);
+}
+template <class R, int FAM, int GEOM> constexpr int tiled_wg() { return 64 * 4 * tiled_waves<R, FAM, GEOM>(); }
 
 __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) {
     int lo = 0, hi = n;
@@ -726,7 +731,7 @@ __device__ __forceinline__ int find_patch(const PatchDesc *p, int n, int64_t g) 
 }
 
 template <class R, bool MIXED, int GEOM, int FAM>
-__global__ void __launch_bounds__((tiled_wg<R, FAM>())) __attribute__((amdgpu_waves_per_eu(path_waves_f<R, PDE_VARCOEF_LUT, FAM>())))
+__global__ void __launch_bounds__((tiled_wg<R, FAM, GEOM>())) __attribute__((amdgpu_waves_per_eu(tiled_waves<R, FAM, GEOM>())))
 path_kernel_tiled(PathKernelArgs a) {
     extern __shared__ float4 tile[];
     __shared__ int s_patch;

@@ -188,7 +188,7 @@ struct PathsArgs {
 // Launchers (defined in engine_f64.hip / engine_f32.hip)
 hipError_t launch_path_tiled_f64(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
 hipError_t launch_path_tiled_f32(int geom, bool mixed, int grid, size_t lds, hipStream_t s, const PathKernelArgs &a);
-int path_tiled_wg(int precision, int step);   // block size of the LDS-tiled kernel
+int path_tiled_wg(int precision, int step, int geom);   // block size of the LDS-tiled kernel
 hipError_t launch_path_f64(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,
                            const PathKernelArgs &a);
 hipError_t launch_path_f32(int pde, int geom, bool persistent, bool mixed, int grid, int wg, hipStream_t s,

@@ -58,6 +58,15 @@ def test_ragged_counts_bit_exact(eng, plain):
         assert np.array_equal(out["Gvar"][a:b], ref["Gvar"], equal_nan=True), (k, c)
         assert out["B"][k] == ref["B"] and out["BB"][k] == ref["BB"], (k, c)
     assert out["counters"]["paths"] == int(N[mc].sum())
+    # the call's step count is the oracle's over every knot's paths: with 3 paths on most knots
+    # a wave's lanes span many knots, whose step counters then take one update per lane
+    pb = O.problem(O.PDE_MONEGROS)
+    steps = 0
+    for k in mc:
+        st = m.stencil(int(k))
+        p = O.simulate(pb, O.run_params(FIX_H, 17, threads=8), m.x0[k], m.y0[k], st["rect"], int(k), 0, int(N[k]))
+        steps += int(p["steps"].sum())
+    assert out["counters"]["path_steps"] == steps
 
 
 def test_step_cap_bit_exact(eng):
